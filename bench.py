@@ -1,0 +1,133 @@
+"""Throughput of the per-step MD path on one MI355X.
+
+Workload: the 998,784-bead brush channel (box 102*sqrt2 x 102*sqrt2 x 60,
+2 x 102^2 grafted chains of 32 at grafting density 0.5, 332,928 solvent,
+LJ rc 2.5 / brush-solvent WCA, DPD gamma 4.5 rc 1, kT 1, dt 0.005, 9-3
+walls), synthetic initial state (seeded), DPD thermostat on, cells rebuilt
+on the skin criterion (skin 0.4).  A step is one full md_main iteration:
+integrate, cell maintenance, LJ+DPD pairs, bonds, walls, second kick.
+
+value = beads x steps / seconds of the mixed-precision path (FP32 pair
+arithmetic, FP64 integration and sums); the plain FP64 path is timed the same
+way and reported beside it.  The pair kernel's average launch duration comes
+from HIP events on the library's stream over the timed steps.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_PAIR = 79          # LJ 41 + DPD 38 per interacting pair (thermostat on)
+PEAK_FP32 = 157.3e12         # MI355X vector FP32
+PEAK_FP64 = 78.6e12          # MI355X vector FP64
+HBM_PEAK = 8.0e12
+
+
+def run_gpu(precision, steps, warmup, p, t, r0, v):
+    from mfa_brush_amd.sim import Simulation
+    s = Simulation(p, precision=precision)
+    s.set_state(r0, v, t)
+    s.md_steps(warmup, graph=True)
+    s.synchronize()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.md_steps(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    pair_ms, pair_n = s.kernel_time("pair")
+    tot = {k: s.kernel_time(k) for k in ("vv1", "rebuild", "pair", "bonded")}
+    s.set_timing(False)
+    npairs = s.pair_count()
+    fl = s.flags()
+    e = s.energies()
+    s.close()
+    return dict(seconds=dt, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl, energies=e,
+                kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+
+
+def cpu_baseline(p, t, r0, v, n_steps):
+    """The oracle's OpenMP build (same algorithm as the reference: linked
+    cells, half shell, full-array force reduction, serial integrator) on
+    this host's cores, on a bounded number of steps of the same workload."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    o = Oracle(p, omp=True)
+    threads = o.lib.orc_num_threads()
+    t0 = time.perf_counter()
+    o.md_steps(r0, v, np.zeros_like(r0), t, 0, n_steps, noise=1)
+    dt = time.perf_counter() - t0
+    return dict(value=p.n_part * n_steps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample="%d steps of the full %d-bead workload (%.1f s)" % (n_steps, p.n_part, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mfa_brush_amd import configs
+    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=0.4)
+    n = p.n_part
+    print("workload: %d beads, box %s" % (n, p.boundary), file=sys.stderr, flush=True)
+
+    mixed = run_gpu("mixed", args.steps, args.warmup, p, t, r0, v)
+    print("mixed: %.3f s for %d steps" % (mixed["seconds"], args.steps), file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_gpu("fp64", args.steps, args.warmup, p, t, r0, v)
+        print("fp64: %.3f s for %d steps" % (fp64["seconds"], args.steps), file=sys.stderr, flush=True)
+
+    value = n * args.steps / mixed["seconds"]
+    pair_avg_s = mixed["pair_ms"] / max(mixed["pair_n"], 1) / 1e3
+    achieved = FLOPS_PER_PAIR * mixed["npairs"] / pair_avg_s
+    out = {
+        "metric": "particle-steps/s @1M beads, mixed vs plain FP64 GPU; time-to-equilibrium",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-pair/fp64-integrate",
+        "data": "synthetic (seeded column-lattice start, Maxwell-Boltzmann velocities)",
+        "config": {"workload": "brush channel 998,784 beads, box 144.25x144.25x60, 2x10404 chains N=32, "
+                               "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005, skin 0.4",
+                   "n_beads": n, "interacting_pairs": mixed["npairs"]},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_pair32 (VALU FP32; no MFMA on the pair path)",
+                     "flops_per_launch": FLOPS_PER_PAIR * mixed["npairs"], "avg_launch_us": pair_avg_s * 1e6},
+        "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in mixed["kernels"].items()},
+    }
+    if fp64 is not None:
+        v64 = n * args.steps / fp64["seconds"]
+        p64 = fp64["pair_ms"] / max(fp64["pair_n"], 1) / 1e3
+        out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+                       "pair_avg_us": p64 * 1e6,
+                       "pair_frac_fp64_peak": FLOPS_PER_PAIR * fp64["npairs"] / p64 / PEAK_FP64,
+                       "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in fp64["kernels"].items()}}
+        out["mixed_over_fp64"] = value / v64
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, t, r0, v, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * mfa_hip.h -- C ABI of the MI355X (gfx950) per-step MD path of mfa_brush.
+ *
+ * Drop-in boundary: each entry point replaces one Fortran subroutine that
+ * md_main.f90 calls on the step path (citations are file:line in the
+ * reference).  Plain pointers and sizes only; arrays use the reference's
+ * `module commons` layout (mfa_common.f90:98-324):
+ *   r0, v, a, force  real(8) (3, n_part) column-major  -> double[3*n_part]
+ *   a_type           integer (n_part), values 1..4      -> int32_t[n_part]
+ *   n_type tables    real(8) (4, 4) column-major         -> double[16]
+ * Bead order is the reference's: n_chain grafted chains of n_mon beads
+ * (head first), then n_chain_d melt/solvent chains of n_mon_d beads.
+ *
+ * Every call returns 0 on success or a negative MFA_E_* code; the text of
+ * the last error is in mfa_hip_last_error().  A context is used from one
+ * host thread (the reference calls the step routines from its master
+ * thread, outside any OpenMP region).  One context holds one or more
+ * independent channels ("systems", i.e. replicas) advanced together.
+ */
+#ifndef MFA_HIP_H
+#define MFA_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFA_E_ARG     (-1)   /* invalid argument / unsupported configuration */
+#define MFA_E_HIP     (-2)   /* HIP runtime error */
+#define MFA_E_STATE   (-3)   /* call out of order (e.g. no state uploaded) */
+#define MFA_E_NOMEM   (-4)
+
+#define MFA_PREC_FP64  0     /* all pair arithmetic in FP64 */
+#define MFA_PREC_MIXED 1     /* FP32 pair arithmetic on cell-relative positions, FP64 integration and sums */
+
+/* Parameters of one channel.  Tables are the ones init_params.f90:393-887
+ * derives; the flags are the hot-path subset of control_simulation.h. */
+typedef struct mfa_system_params {
+    int32_t n_mon, n_chain;       /* grafted chains (init_system.f90:33) */
+    int32_t n_mon_d, n_chain_d;   /* melt / solvent chains (init_system.f90:36) */
+    int32_t thermostat;           /* 1: DPD pair forces (THERMOSTAT==0, control_simulation.h:59) */
+    int32_t dpd_cut_off;          /* 1: DPD_CUT_OFF defined (control_simulation.h:63) */
+    int32_t force_switch_on;      /* FORCE_SWITCH_ON (control_simulation.h:73) */
+    int32_t bending;              /* BENDING (control_simulation.h:47) */
+    int32_t bending_melt;         /* BENDING_MELT (control_simulation.h:48) */
+    int32_t poiseuille;           /* POISEUILLE (control_simulation.h:80) */
+    int32_t l_box;                /* L_BOX 1 or 2 (control_simulation.h:105) */
+    int32_t reserved0;
+    double boundary[3];           /* box; z = z_space_wall (init_config.f90:17-19) */
+    double epsil[16];             /* 4*epsilon, epsil(1,1)=0 */
+    double sigma_2[16];
+    double range_2[16];           /* cutoff^2, 0 disables the pair */
+    double e_shift[16];
+    double inv_range_2[16];
+    double r_cut_max;
+    double r_cut_dpd_2;           /* used when dpd_cut_off */
+    double skin;                  /* 0: cells of r_cut_max rebuilt every step (BIN_TYPE 2);
+                                     >0: cells of r_cut_max+skin rebuilt when a bead moved
+                                     more than skin/2 (check_skin.f90:11-24) */
+    double sig;                   /* sqrt(2 kT gamma/dt) (init_params.f90:887) */
+    double friction;              /* gamma = friction(1) (dpd_forces.f90:105) */
+    double r_2_min_init, r_2_min_time;   /* force switch-on ramp (init_force_switch_on.f90:14-21) */
+    double sigma_wall[4], a_wall[4];     /* init_params.f90:593-596 */
+    double a_w;
+    double k_chain;               /* k * R0^2 (init_params.f90:396) */
+    double inv_r_chain_2;
+    double k_bend, alpha_eq;
+    double dt;
+    double const_force[3];        /* Poiseuille body force g (init_config.f90:370-378) */
+    double head_vel[2];           /* va_spring_twall(1:2) for heads when !poiseuille */
+    double mass_type[4];
+    uint64_t seed;                /* Philox key of the DPD noise */
+    int64_t t_idx0;               /* time index of the next step: r_time = t_idx*dt */
+} mfa_system_params;
+
+typedef struct mfa_energies {
+    double v_fluid_fluid;   /* fluid_fluid.f90:137 */
+    double v_fluid_wall;    /* fluid_wall.f90:98,116 */
+    double v_intra_molec;   /* intra_molec.f90:72,129,266 */
+    double v_bend;          /* bending.f90:100 */
+    double v_bend_melt;     /* bending_melt.f90 */
+    double t_fluid;         /* 0.5 sum m v^2, verlet_velocities.f90:132-153 */
+    double virial[6];       /* pair virial W = sum r_ij (x) F_ij, xx yy zz xy xz yz (no reference counterpart) */
+} mfa_energies;
+
+typedef struct mfa_flags {
+    int32_t nonfinite;       /* beads with NaN/Inf position */
+    int32_t out_of_box_z;    /* beads with z outside [0, Lz] (fluid_wall.f90:86,106 warnings) */
+    int32_t fene_stretched;  /* bonds clamped at 0.98 R0^2 (intra_molec.f90:39-43) */
+    int32_t rebuilds;        /* cell-list rebuilds since creation */
+} mfa_flags;
+
+typedef struct mfa_ctx mfa_ctx;
+
+/* ---- lifetime and state -------------------------------------------------
+ * mfa_hip_create allocates device buffers for n_sys channels (replaces the
+ * allocation of init_system.f90:196-287 on the device side). */
+int  mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precision, mfa_ctx **out);
+int  mfa_hip_destroy(mfa_ctx *ctx);
+/* host -> device copy of one channel's state (conf_old / gen_* output,
+ * init_config.f90:164-173).  a may be NULL (zero).  Builds the cell list
+ * (make_binning + neigh_list + get_cell_neigh_list, init_config.f90:261-272). */
+int  mfa_hip_set_state(mfa_ctx *ctx, int32_t isys, const double *r0, const double *v,
+                       const double *a, const int32_t *a_type);
+/* device -> host, reference bead order; any pointer may be NULL.
+ * Used at the host-consumer sites observation / store_config / obser_out
+ * (md_main.f90:202-217). */
+int  mfa_hip_get_state(mfa_ctx *ctx, int32_t isys, double *r0, double *v, double *a, double *force);
+int  mfa_hip_set_time(mfa_ctx *ctx, int64_t t_idx);   /* r_time = t_idx*dt (md_main.f90:79) */
+int  mfa_hip_get_time(mfa_ctx *ctx, int64_t *t_idx);
+
+/* ---- step routines (drop-in for md_main.f90:85-185) -------------------- */
+int  mfa_hip_verlet_positions(mfa_ctx *ctx);       /* verlet_positions.f90:150-209 + cell relink/rebuild */
+int  mfa_hip_check_skin(mfa_ctx *ctx, int32_t *n_rebuilds); /* check_skin.f90:4-28: the skin test runs inside
+                                                      verlet_positions; this reports the rebuild count */
+int  mfa_hip_fluid_fluid(mfa_ctx *ctx);            /* fluid_fluid.f90:1-512 + dpd_forces.f90; also
+                                                      performs md_main.f90:107 force(:,:)=0 */
+int  mfa_hip_fluid_wall(mfa_ctx *ctx, int32_t inter_type);   /* fluid_wall.f90:77-123 (inter_type 2) */
+int  mfa_hip_wall_wall(mfa_ctx *ctx, int32_t inter_type, double sigma_w, double *v_wall_wall); /* wall_wall.f90:45-47 */
+int  mfa_hip_intra_molec(mfa_ctx *ctx);            /* intra_molec.f90:1-268 */
+int  mfa_hip_bending(mfa_ctx *ctx, int32_t mode);  /* bending.f90:19-121 (mode 1) */
+int  mfa_hip_bending_melt(mfa_ctx *ctx, int32_t mode);   /* bending_melt.f90:22-120 (mode 1) */
+int  mfa_hip_constant_force(mfa_ctx *ctx, int32_t case_pois); /* constant_force.f90:17-21 (case 1) */
+int  mfa_hip_verlet_velocities(mfa_ctx *ctx);      /* verlet_velocities.f90:18-153; ends the step */
+
+/* Fused steps: n_steps of md_main.f90:77-185 in 4 kernels per step
+ * (integrate+skin check, [gated cell rebuild], pair, bonded+wall+kick).
+ * use_graph=1 captures one step in a hipGraph and replays it. */
+int  mfa_hip_md_steps(mfa_ctx *ctx, int32_t n_steps, int32_t use_graph);
+int  mfa_hip_synchronize(mfa_ctx *ctx);
+
+/* ---- observables and checks ------------------------------------------- */
+int  mfa_hip_get_energies(mfa_ctx *ctx, int32_t isys, mfa_energies *out);
+int  mfa_hip_get_f_on_heads(mfa_ctx *ctx, int32_t isys, double *f_on_heads /* (3, n_chain) */);
+int  mfa_hip_get_flags(mfa_ctx *ctx, mfa_flags *out);
+/* check_fluid_fluid, util.f90:284-330: O(N^2) brute-force LJ energy */
+int  mfa_hip_check_fluid_fluid(mfa_ctx *ctx, int32_t isys, double *v_fluid_fluid);
+/* interacting-pair fingerprint per bead (reference order): number of partners
+ * with r^2 < range_2 and the wrapping sum of mix64(partner index) */
+int  mfa_hip_pair_stats(mfa_ctx *ctx, int32_t isys, int32_t *count, uint64_t *hash);
+/* histo(2) functions.f90:157-213 and velocity_prof(2) :319-365, accumulated
+ * on the device: dens_z (n_bins,2) brush/melt counts; v_prof, v_prof_2
+ * (n_bins,3,2) sums of v and v^2 per bin, species brush/melt. */
+int  mfa_hip_profiles_reset(mfa_ctx *ctx, int32_t n_bins);
+int  mfa_hip_profiles_accumulate(mfa_ctx *ctx);
+int  mfa_hip_profiles_get(mfa_ctx *ctx, int32_t isys, int32_t *n_obs, double *dens_z,
+                          double *v_prof, double *v_prof_2);
+/* make_binning, cell_list.f90:129-158 (host): cells for a cutoff */
+int  mfa_hip_make_binning(int32_t bin_case, const double *boundary, double r_cut, double *l_cell,
+                          double *inv_l_cell, int32_t *n_cells, int32_t *n_cells_tot, int32_t *n_nei);
+/* the device cell list of one channel: cell_start/cell_count (n_cells_tot)
+ * and the bead (reference index, 0-based) in each cell slot (n_part) */
+int  mfa_hip_get_cells(mfa_ctx *ctx, int32_t isys, int32_t *n_cells, int32_t *cell_start,
+                       int32_t *cell_count, int32_t *cell_beads);
+
+/* ---- timing ------------------------------------------------------------
+ * When enabled, each fused-step kernel launch is bracketed by hipEvents on
+ * the context's stream; totals are in milliseconds. */
+int  mfa_hip_set_timing(mfa_ctx *ctx, int32_t on);
+int  mfa_hip_get_kernel_time(mfa_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off) */
+int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* interacting pairs, last evaluation */
+
+const char *mfa_hip_last_error(void);
+int  mfa_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

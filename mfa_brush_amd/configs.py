@@ -1,0 +1,183 @@
+"""Synthetic channel configurations and the named workloads.
+
+The reference builds initial states with gen_brush.f90 (grafted chains on a
+square lattice, mode 5 ordered, :362-441) and gen_droplet.f90 (melt/solvent
+random walks, :18-69) and relaxes overlaps with the force switch-on ramp.
+Here the initial state is overlap-free by construction: every bead sits on a
+column site (columns on a square lattice, sites 0.97 sigma apart in z, close
+to the FENE+LJ bond length), grafted chains are straight columns from their
+heads at z=1.2 (bottom wall) and z=Lz-1.2 (top wall, lattice offset by half a
+cell so the two brushes interleave), and solvent beads take randomly chosen
+free column sites.  Velocities are Maxwell-Boltzmann at kT, heads at rest.
+
+Workloads (sizes as stated, seeds fixed):
+  * channel_c1:  box 16x16x20, 2x64 chains of 16 on 8x8 lattices (a=2),
+    2048 solvent -> 4096 beads; LJ rc=2.5 (brush-brush, solvent-solvent,
+    head-solvent), brush-solvent WCA, DPD gamma=4.5 rc_DPD=1, kT=1, dt=0.005.
+  * channel_c2:  box 144.25x144.25x60 (102 sqrt2 square), 2x102^2 chains of
+    32 (a=sqrt2, grafting density 0.5), 332,928 solvent -> 998,784 beads,
+    same potentials.
+  * melt_c3:     1.2M melt beads in 18,750 chains of 64, box 216.93^2 x 30,
+    harmonic-angle bending k=5, Poiseuille g=0.01 x, DPD.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import replace
+from typing import Optional
+
+import numpy as np
+
+from .params import Flags, SimParams, TWO_SIXTH, f2d
+
+BOND_Z = 0.97
+
+
+def _tables(rc: np.ndarray, eps: np.ndarray, sig: np.ndarray):
+    """Fortran-ordered tables from per-type-pair cutoffs (rc=0: off),
+    epsilon and sigma, with the shift of init_params.f90:726-737."""
+    range_2 = rc ** 2
+    sigma_2 = sig ** 2
+    epsil = 4.0 * eps
+    e_shift = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(4):
+            if range_2[i, j] > 0.0:
+                rd = (sigma_2[i, j] / range_2[i, j]) ** 3
+                e_shift[i, j] = rd * (rd - 1.0)
+    inv = np.where(range_2 > 0, 1.0 / np.where(range_2 > 0, range_2, 1.0), 0.0)
+    col = lambda m: np.ascontiguousarray(m.T.reshape(-1))
+    return col(epsil), col(sigma_2), col(range_2), col(e_shift), col(inv), float(math.sqrt(range_2.max()))
+
+
+def channel_params(boundary, n_mon, n_chain, n_solvent, *, rc_lj=2.5, gamma=4.5, kT=1.0, dt=0.005,
+                   rc_dpd: Optional[float] = 1.0, thermostat=True, a_w=1.0, sigma_w=1.0, seed=0x5EED0001,
+                   eps_bs=1.0, skin=0.0, l_box=1, bending=False, k_bend=0.0, const_force=(0.0, 0.0, 0.0),
+                   force_switch_on=False) -> SimParams:
+    """Brush-in-channel parameters: type 1 heads, 2 brush, 3 solvent, 4 off.
+    LJ cutoff rc_lj for brush-brush, solvent-solvent, head-brush and
+    head-solvent; brush-solvent WCA (float 2^(1/6)) with epsilon eps_bs;
+    head-head off (init_params.f90:719)."""
+    rc = np.zeros((4, 4))
+    eps = np.ones((4, 4))
+    sig = np.ones((4, 4))
+    wca = TWO_SIXTH
+    for (a, b, r) in ((0, 1, rc_lj), (1, 1, rc_lj), (2, 2, rc_lj), (0, 2, rc_lj), (1, 2, wca)):
+        rc[a, b] = rc[b, a] = r
+    eps[1, 2] = eps[2, 1] = eps_bs
+    eps[0, 0] = 0.0
+    epsil, sigma_2, range_2, e_shift, inv, rcmax = _tables(rc, eps, sig)
+    fl = Flags(solvent=3, thermostat=thermostat, dpd_cut_off=rc_dpd, force_switch_on=force_switch_on,
+               poiseuille=any(c != 0.0 for c in const_force), bending=bending, l_box=l_box)
+    r_cut_dpd_2 = 0.0
+    if rc_dpd is not None:
+        r = f2d(rc_dpd)
+        r_cut_dpd_2 = r * r
+        rcmax = max(rcmax, r)
+    mass = np.array([1.0, 1.0, 1.0, 1.0])
+    return SimParams(
+        n_mon=n_mon, n_chain=n_chain, n_mon_d=1, n_chain_d=n_solvent, boundary=tuple(boundary),
+        epsil=epsil, sigma_2=sigma_2, range_2=range_2, e_shift=e_shift, inv_range_2=inv,
+        r_cut_max=rcmax, r_cut_dpd_2=r_cut_dpd_2, sig=math.sqrt(2.0 * kT * gamma / dt), friction=gamma,
+        r_2_min_init=(f2d(0.9) * 1.0) ** 2, r_2_min_time=40.0,
+        sigma_wall=np.array([(sigma_w + 1.0) / 2] * 4), a_wall=np.array([a_w] * 4), a_w=a_w, sigma_w=sigma_w,
+        k_chain=30.0 * 1.5 ** 2, inv_r_chain_2=1.0 / 1.5 ** 2, k_bend=k_bend, alpha_eq=0.0, dt=dt,
+        mass_type=mass, const_force=tuple(const_force), seed=seed, flags=fl, skin=skin, temp=kT)
+
+
+def _columns(L, a, offset_frac, n_side):
+    g = (np.arange(n_side) + offset_frac) * a
+    xx, yy = np.meshgrid(g, g, indexing="ij")
+    return np.stack([xx.ravel(), yy.ravel()], 1) % L
+
+
+def build_channel(p: SimParams, a_graft: float, n_side: int, rng_seed: int = 12345, kT: float = 1.0):
+    """Initial state (a_type, r0, v) for a two-brush channel in reference
+    bead order: bottom chains, top chains (heads first), then solvent."""
+    rng = np.random.default_rng(rng_seed)
+    L, Ly, Lz = p.boundary
+    assert abs(L - Ly) < 1e-9
+    n_chain_half = n_side * n_side
+    assert p.n_chain == 2 * n_chain_half
+    bot = _columns(L, a_graft, 0.5, n_side)
+    top = _columns(L, a_graft, 0.0, n_side)
+    cols = [bot, top]
+    if a_graft / 2 >= 0.95:   # extra solvent columns between brush columns
+        cols.append((bot + np.array([a_graft / 2, 0.0])) % L)
+        cols.append((bot + np.array([0.0, a_graft / 2])) % L)
+    z_lo, z_hi = 1.2, Lz - 1.2
+    n_site = int(math.floor((z_hi - z_lo) / BOND_Z)) + 1
+    zs_up = z_lo + BOND_Z * np.arange(n_site)
+    zs_dn = z_hi - BOND_Z * np.arange(n_site)
+    n_mon = p.n_mon
+    r0 = np.zeros((p.n_part, 3))
+    a_type = np.zeros(p.n_part, np.int32)
+    k = 0
+    for side, (xy, zs) in enumerate(((bot, zs_up), (top, zs_dn))):
+        for c in range(n_chain_half):
+            r0[k:k + n_mon, 0] = xy[c, 0]
+            r0[k:k + n_mon, 1] = xy[c, 1]
+            r0[k:k + n_mon, 2] = zs[:n_mon]
+            a_type[k] = 1
+            a_type[k + 1:k + n_mon] = 2
+            k += n_mon
+    # free column sites (bottom-brush columns above the chains, top-brush
+    # columns below them, solvent-only columns whole)
+    site_runs = []
+    for ci, xy in enumerate(cols):
+        zs = zs_up if ci != 1 else zs_dn
+        start = n_mon if ci < 2 else 0
+        zz = zs[start:]
+        zz = zz[(zz > 0.6) & (zz < Lz - 0.6)]
+        for c in range(len(xy)):
+            site_runs.append(np.stack([np.full(len(zz), xy[c, 0]), np.full(len(zz), xy[c, 1]), zz], 1))
+    n_sol = p.n_chain_d * p.n_mon_d
+    if p.n_mon_d == 1:
+        free = np.concatenate(site_runs)
+        if n_sol > len(free):
+            raise ValueError("not enough free sites for %d solvent beads (have %d)" % (n_sol, len(free)))
+        pick = rng.choice(len(free), size=n_sol, replace=False)
+        r0[k:k + n_sol] = free[np.sort(pick)]
+    else:
+        # melt chains: runs of n_mon_d consecutive sites of one column
+        runs = [(ri, q) for ri, run in enumerate(site_runs) for q in range(len(run) // p.n_mon_d)]
+        if p.n_chain_d > len(runs):
+            raise ValueError("not enough column runs for %d melt chains" % p.n_chain_d)
+        pick = np.sort(rng.choice(len(runs), size=p.n_chain_d, replace=False))
+        for c, idx in enumerate(pick):
+            ri, q = runs[idx]
+            r0[k + c * p.n_mon_d:k + (c + 1) * p.n_mon_d] = site_runs[ri][q * p.n_mon_d:(q + 1) * p.n_mon_d]
+    a_type[k:k + n_sol] = 3
+    # small jitter (keeps the minimum distance above 0.9) and wrap
+    r0[:, :2] += rng.uniform(-0.02, 0.02, size=(p.n_part, 2))
+    r0[:, 0] %= L
+    r0[:, 1] %= Ly
+    v = rng.normal(0.0, math.sqrt(kT), size=(p.n_part, 3))
+    heads = np.arange(p.n_chain) * n_mon
+    v[heads] = 0.0
+    return a_type, r0, v
+
+
+def channel_c1(**kw):
+    p = channel_params((16.0, 16.0, 20.0), 16, 128, 2048, **kw)
+    a_type, r0, v = build_channel(p, 2.0, 8)
+    return p, a_type, r0, v
+
+
+def channel_c2(**kw):
+    a = math.sqrt(2.0)
+    L = 102 * a
+    p = channel_params((L, L, 60.0), 32, 2 * 102 * 102, 332928, **kw)
+    a_type, r0, v = build_channel(p, a, 102)
+    return p, a_type, r0, v
+
+
+def channel_scaled(n_side: int, n_mon: int = 32, lz: float = 60.0, density: float = 0.8, **kw):
+    """C2 geometry with n_side^2 chains per wall (C2 is n_side=102)."""
+    a = math.sqrt(2.0)
+    L = n_side * a
+    n_brush = 2 * n_side * n_side * n_mon
+    n_sol = int(round(density * L * L * lz)) - n_brush
+    p = channel_params((L, L, lz), n_mon, 2 * n_side * n_side, n_sol, **kw)
+    a_type, r0, v = build_channel(p, a, n_side)
+    return p, a_type, r0, v

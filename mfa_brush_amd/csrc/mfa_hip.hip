@@ -1,0 +1,803 @@
+// mfa_hip.hip -- C ABI (include/mfa_hip.h) and host orchestration of the
+// gfx950 per-step path.  One HIP stream per context; every kernel of a step
+// is enqueued on it, and the cell-list rebuild is decided on the device (a
+// flag written by the integrator and read by the gated rebuild kernels), so
+// a step needs no host round trip and can be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mfa_hip.h"
+#include "mfa_kernels.h"
+#include "mfa_pair.h"
+#include "mfa_step.h"
+
+using namespace mfa;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(MFA_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+namespace {
+
+constexpr int GATED_GRID = 1024;     // grid of the rebuild kernels (grid-stride)
+
+struct TimedLaunch {
+    hipEvent_t a, b;
+    int kernel;
+};
+
+enum KId { K_VV1 = 0, K_REBUILD, K_PAIR, K_BONDED, K_NK };
+const char *kNames[K_NK] = {"vv1", "rebuild", "pair", "bonded"};
+
+}  // namespace
+
+struct mfa_ctx {
+    int n_sys = 0, prec = 0;
+    std::vector<mfa_system_params> hp;
+    std::vector<SysDev> hs;
+    int S_tot = 0, N_tot = 0, C_tot = 0, n_blocks = 0, H_tot = 0;
+    hipStream_t st = nullptr;
+    SysDev *d_sys = nullptr;
+    int *d_blk_sys = nullptr, *d_cell_sys = nullptr;
+    Bufs B{};
+    int *d_cell_count = nullptr, *d_pre = nullptr, *d_bsum = nullptr;
+    int2 *d_crange = nullptr;
+    int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr;
+    double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
+    int *d_rebuild_count = nullptr;
+    double *d_stage = nullptr;
+    int *d_istage = nullptr;
+    // profiles
+    int prof_nb = 0, prof_nobs = 0;
+    unsigned int *d_pcnt = nullptr;
+    unsigned long long *d_pvs = nullptr, *d_pv2s = nullptr;
+    double *d_pdens = nullptr, *d_pvp = nullptr, *d_pvp2 = nullptr;
+    bool have_state = false;
+    std::vector<char> uploaded;
+    bool virial = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<TimedLaunch> pending;
+    double k_ms[K_NK] = {0, 0, 0, 0};
+    long long k_n[K_NK] = {0, 0, 0, 0};
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+static hipEvent_t take_event(mfa_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Timer {
+    mfa_ctx *c;
+    int kid;
+    hipEvent_t a = nullptr;
+    Timer(mfa_ctx *c_, int k) : c(c_), kid(k) {
+        if (c->timing) { a = take_event(c); if (a) (void)hipEventRecord(a, c->st); }
+    }
+    ~Timer() {
+        if (c->timing && a) {
+            hipEvent_t b = take_event(c);
+            if (b) { (void)hipEventRecord(b, c->st); c->pending.push_back({a, b, kid}); }
+        }
+    }
+};
+
+static int collect_timing(mfa_ctx *c) {
+    if (c->pending.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (auto &t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->k_ms[t.kernel] += ms;
+            c->k_n[t.kernel] += 1;
+        }
+        c->ev_pool.push_back(t.a);
+        c->ev_pool.push_back(t.b);
+    }
+    c->pending.clear();
+    return 0;
+}
+
+template <class T>
+static int dalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+    HIPCHK(hipMemset(*p, 0, n * sizeof(T)));
+    return 0;
+}
+
+static void free_all(mfa_ctx *c) {
+    auto F = [](void *p) { if (p) (void)hipFree(p); };
+    for (int h = 0; h < 2; ++h) {
+        F(c->B.pos[h]); F(c->B.vel[h]); F(c->B.acc[h]); F(c->B.posb[h]);
+        F(c->B.posf[h]); F(c->B.velf[h]); F(c->B.cell_of[h]);
+    }
+    F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.flags); F(c->B.t_idx);
+    F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
+    F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_partial);
+    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
+    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->st) (void)hipStreamDestroy(c->st);
+}
+
+// make_binning, cell_list.f90:129-158
+static void make_binning(int bin_case, const double *boundary, double r_cut, double *l_cell, double *inv_l_cell,
+                         int *n_cells, int *n_cells_tot, int *n_nei) {
+    for (int d = 0; d < 3; ++d)
+        n_cells[d] = bin_case == 2 ? (int)(boundary[d] / (r_cut / 2.0)) : (int)(boundary[d] / r_cut);
+    *n_nei = bin_case == 2 ? 62 : 13;
+    int tot = 1;
+    for (int d = 0; d < 3; ++d) {
+        l_cell[d] = boundary[d] / (double)n_cells[d];
+        inv_l_cell[d] = 1.0 / l_cell[d];
+        tot *= n_cells[d];
+    }
+    *n_cells_tot = tot;
+}
+
+static int build_sysdev(const mfa_system_params &p, SysDev &d) {
+    memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; ++k) {
+        if (!(p.boundary[k] > 0.0)) return fail(MFA_E_ARG, "boundary must be positive");
+        d.L[k] = p.boundary[k];
+        d.invL[k] = 1.0 / p.boundary[k];
+    }
+    if (p.l_box != 1 && p.l_box != 2) return fail(MFA_E_ARG, "l_box must be 1 or 2");
+    if (p.n_mon < 0 || p.n_chain < 0 || p.n_mon_d < 0 || p.n_chain_d < 0)
+        return fail(MFA_E_ARG, "negative chain counts");
+    if (p.n_chain > 0 && p.n_mon < 1) return fail(MFA_E_ARG, "n_mon must be >= 1 with grafted chains");
+    if (p.n_chain_d > 0 && p.n_mon_d < 1) return fail(MFA_E_ARG, "n_mon_d must be >= 1 with melt chains");
+    if (!(p.dt > 0.0)) return fail(MFA_E_ARG, "dt must be positive");
+    if (p.skin < 0.0) return fail(MFA_E_ARG, "skin must be >= 0");
+    const double rc = p.r_cut_max + p.skin;
+    if (!(rc > 0.0)) return fail(MFA_E_ARG, "r_cut_max + skin must be positive");
+    int nnei = 0, nct = 0;
+    make_binning(p.l_box, p.boundary, rc, d.l_cell, d.inv_l_cell, d.nc, &nct, &nnei);
+    d.S = p.l_box == 2 ? 2 : 1;
+    // The half-shell stencil of cell_list.f90:345-402 repeats a cell when a
+    // periodic direction has fewer than 2S+1 cells (the reference then
+    // double counts pairs); such boxes are rejected.
+    if (d.nc[0] < 2 * d.S + 1 || d.nc[1] < 2 * d.S + 1)
+        return fail(MFA_E_ARG, "box too small for the cell stencil: need >= 2*L_BOX+1 cells in x and y");
+    if (d.nc[2] < 1) return fail(MFA_E_ARG, "box shorter than one cell in z");
+    d.n_cells_tot = nct;
+    for (int t = 0; t < 16; ++t) {
+        d.epsil[t] = p.epsil[t]; d.sigma_2[t] = p.sigma_2[t]; d.range_2[t] = p.range_2[t];
+        d.e_shift[t] = p.e_shift[t]; d.inv_range_2[t] = p.inv_range_2[t];
+        d.epsil_f[t] = (float)p.epsil[t]; d.sigma_2f[t] = (float)p.sigma_2[t];
+        d.range_2f_lo[t] = (float)(p.range_2[t] * (1.0 - 1.0 / 16384.0));
+        d.range_2f_hi[t] = (float)(p.range_2[t] * (1.0 + 1.0 / 16384.0));
+        d.e_shift_f[t] = (float)p.e_shift[t];
+        d.inv_rc_f[t] = (float)sqrt(p.inv_range_2[t]);
+    }
+    d.r_cut_dpd_2 = p.r_cut_dpd_2;
+    if (p.dpd_cut_off && !(p.r_cut_dpd_2 > 0.0)) return fail(MFA_E_ARG, "dpd_cut_off needs r_cut_dpd_2 > 0");
+    if (p.dpd_cut_off && sqrt(p.r_cut_dpd_2) > rc) return fail(MFA_E_ARG, "DPD cutoff exceeds the cell size");
+    d.sig = p.sig; d.friction = p.friction;
+    d.r_2_min_init = p.r_2_min_init; d.r_2_min_time = p.r_2_min_time;
+    for (int t = 0; t < 4; ++t) {
+        d.sigma_wall[t] = p.sigma_wall[t]; d.a_wall[t] = p.a_wall[t];
+        d.mass_type[t] = p.mass_type[t];
+        d.inv_mass_type[t] = p.mass_type[t] != 0.0 ? 1.0 / p.mass_type[t] : 0.0;
+    }
+    d.a_w = p.a_w;
+    d.k_chain = p.k_chain; d.inv_r_chain_2 = p.inv_r_chain_2; d.k_bend = p.k_bend; d.alpha_eq = p.alpha_eq;
+    d.dt = p.dt; d.dt_2 = p.dt * p.dt;
+    for (int k = 0; k < 3; ++k) d.const_force[k] = p.const_force[k];
+    d.head_vel[0] = p.head_vel[0]; d.head_vel[1] = p.head_vel[1];
+    d.skin_mode = p.skin > 0.0 ? 1 : 0;
+    d.skin_half_2 = 0.25 * p.skin * p.skin;
+    d.seed = p.seed;
+    d.n_part = p.n_mon * p.n_chain + p.n_mon_d * p.n_chain_d;
+    d.n_mon = p.n_mon; d.n_chain = p.n_chain; d.n_mon_d = p.n_mon_d; d.n_chain_d = p.n_chain_d;
+    d.pd = p.n_mon * p.n_chain;
+    d.thermostat = p.thermostat; d.dpd_cut_off = p.dpd_cut_off; d.force_switch_on = p.force_switch_on;
+    d.bending = p.bending; d.bending_melt = p.bending_melt; d.poiseuille = p.poiseuille;
+    if (d.n_part < 1) return fail(MFA_E_ARG, "channel without beads");
+    return 0;
+}
+
+template <class K, class... A>
+static void launch(hipStream_t st, K kern, int grid, A... args) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, st, args...);
+}
+
+// ---------------------------------------------------------------------------
+// step pieces
+// ---------------------------------------------------------------------------
+static void enqueue_rebuild(mfa_ctx *c) {
+    Timer t(c, K_REBUILD);
+    const int g = std::min(GATED_GRID, std::max(1, c->n_blocks));
+    const int gc = std::min(GATED_GRID, (c->C_tot + BLK - 1) / BLK);
+    const int nsb = (c->C_tot + 4 * BLK - 1) / (4 * BLK);
+    launch(c->st, k_bin, g, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank, c->S_tot);
+    launch(c->st, k_scan1, nsb, c->B.rebuild, c->d_cell_count, c->C_tot, c->d_pre, c->d_bsum);
+    launch(c->st, k_scan2, 1, c->B.rebuild, c->d_bsum, nsb);
+    launch(c->st, k_scan3, gc, c->d_sys, c->B.rebuild, c->d_cell_count, c->d_pre, c->d_bsum, c->d_cell_sys,
+           c->d_crange, c->C_tot);
+    launch(c->st, k_scatter, g, c->d_sys, c->d_blk_sys, c->B.rebuild, c->d_bin_cell, c->d_bin_rank, c->d_crange,
+           c->d_perm, c->S_tot);
+    launch(c->st, k_cellsort, gc, c->B, c->d_crange, c->d_perm, c->C_tot);
+    if (c->prec == MFA_PREC_MIXED)
+        launch(c->st, k_permute<true>, g, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
+    else
+        launch(c->st, k_permute<false>, g, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
+    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count);
+}
+
+static void enqueue_vv1(mfa_ctx *c) {
+    Timer t(c, K_VV1);
+    if (c->prec == MFA_PREC_MIXED) launch(c->st, k_vv1<true>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B);
+    else launch(c->st, k_vv1<false>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B);
+}
+
+static void enqueue_pair(mfa_ctx *c) {
+    Timer t(c, K_PAIR);
+    bool dpd = false;
+    for (auto &p : c->hp) dpd = dpd || p.thermostat;
+    const int nb = c->n_blocks;
+    if (c->prec == MFA_PREC_MIXED) {
+        if (dpd) {
+            if (c->virial) launch(c->st, k_pair32<true, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else launch(c->st, k_pair32<true, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+        } else {
+            if (c->virial) launch(c->st, k_pair32<false, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else launch(c->st, k_pair32<false, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+        }
+    } else {
+        if (dpd) {
+            if (c->virial) launch(c->st, k_pair64<true, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else launch(c->st, k_pair64<true, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+        } else {
+            if (c->virial) launch(c->st, k_pair64<false, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else launch(c->st, k_pair64<false, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+        }
+    }
+}
+
+template <int MODE>
+static void enqueue_bonded(mfa_ctx *c) {
+    Timer t(c, K_BONDED);
+    if (c->prec == MFA_PREC_MIXED)
+        launch(c->st, k_bonded<MODE, true>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+    else
+        launch(c->st, k_bonded<MODE, false>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+}
+
+static void enqueue_step(mfa_ctx *c) {
+    enqueue_vv1(c);
+    enqueue_rebuild(c);
+    enqueue_pair(c);
+    enqueue_bonded<M_FUSED>(c);
+}
+
+static int check_ctx(mfa_ctx *c, bool need_state = true) {
+    if (!c) return fail(MFA_E_ARG, "null context");
+    if (need_state && !c->have_state) return fail(MFA_E_STATE, "no state uploaded (mfa_hip_set_state)");
+    return 0;
+}
+
+static void invalidate_graph(mfa_ctx *c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *mfa_hip_last_error(void) { return g_err.c_str(); }
+int mfa_hip_version(void) { return 1; }
+
+int mfa_hip_make_binning(int32_t bin_case, const double *boundary, double r_cut, double *l_cell, double *inv_l_cell,
+                         int32_t *n_cells, int32_t *n_cells_tot, int32_t *n_nei) {
+    if (!boundary || !l_cell || !inv_l_cell || !n_cells || !n_cells_tot || !n_nei || !(r_cut > 0.0))
+        return fail(MFA_E_ARG, "make_binning: bad argument");
+    if (bin_case != 1 && bin_case != 2) return fail(MFA_E_ARG, "make_binning: bin_case must be 1 or 2");
+    make_binning(bin_case, boundary, r_cut, l_cell, inv_l_cell, n_cells, n_cells_tot, n_nei);
+    return 0;
+}
+
+int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precision, mfa_ctx **out) {
+    if (!sys || n_sys < 1 || !out) return fail(MFA_E_ARG, "create: bad argument");
+    if (precision != MFA_PREC_FP64 && precision != MFA_PREC_MIXED) return fail(MFA_E_ARG, "create: bad precision");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MFA_E_HIP, "no HIP device");
+    mfa_ctx *c = new mfa_ctx();
+    c->n_sys = n_sys;
+    c->prec = precision;
+    c->hp.assign(sys, sys + n_sys);
+    c->hs.resize(n_sys);
+    c->uploaded.assign(n_sys, 0);
+    long long slots = 0, parts = 0, cells = 0, heads = 0;
+    for (int s = 0; s < n_sys; ++s) {
+        int rc = build_sysdev(sys[s], c->hs[s]);
+        if (rc) { delete c; return rc; }
+        SysDev &d = c->hs[s];
+        d.slot_base = (int)slots;
+        d.part_base = (int)parts;
+        d.cell_base = (int)cells;
+        d.head_base = (int)heads;
+        d.block_base = (int)(slots / BLK);
+        d.n_blocks = (d.n_part + BLK - 1) / BLK;
+        d.shift = (int)(slots - parts);
+        slots += (long long)d.n_blocks * BLK;
+        parts += d.n_part;
+        cells += d.n_cells_tot;
+        heads += d.n_chain;
+    }
+    if (slots >= (1ll << 28) || cells >= (1ll << 30)) { delete c; return fail(MFA_E_ARG, "too many beads or cells"); }
+    c->S_tot = (int)slots; c->N_tot = (int)parts; c->C_tot = (int)cells; c->H_tot = (int)heads;
+    c->n_blocks = c->S_tot / BLK;
+    std::vector<int> blk_sys(c->n_blocks), cell_sys(c->C_tot);
+    for (int s = 0; s < n_sys; ++s) {
+        for (int b = 0; b < c->hs[s].n_blocks; ++b) blk_sys[c->hs[s].block_base + b] = s;
+        for (int q = 0; q < c->hs[s].n_cells_tot; ++q) cell_sys[c->hs[s].cell_base + q] = s;
+    }
+    int rc = 0;
+#define ALLOC(p, n) do { rc = dalloc(&(p), (n)); if (rc) { free_all(c); delete c; return rc; } } while (0)
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c; return fail(MFA_E_HIP, "stream creation failed");
+    }
+    for (int h = 0; h < 2; ++h) {
+        ALLOC(c->B.pos[h], c->S_tot); ALLOC(c->B.vel[h], c->S_tot); ALLOC(c->B.acc[h], c->S_tot);
+        ALLOC(c->B.posb[h], c->S_tot); ALLOC(c->B.cell_of[h], c->S_tot);
+        if (precision == MFA_PREC_MIXED) { ALLOC(c->B.posf[h], c->S_tot); ALLOC(c->B.velf[h], c->S_tot); }
+    }
+    ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, 1); ALLOC(c->B.rebuild, 1);
+    ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1);
+    ALLOC(c->d_sys, n_sys); ALLOC(c->d_blk_sys, c->n_blocks); ALLOC(c->d_cell_sys, c->C_tot);
+    ALLOC(c->d_cell_count, c->C_tot); ALLOC(c->d_pre, c->C_tot);
+    ALLOC(c->d_bsum, (c->C_tot + 4 * BLK - 1) / (4 * BLK) + 1);
+    ALLOC(c->d_crange, c->C_tot);
+    ALLOC(c->d_bin_cell, c->S_tot); ALLOC(c->d_bin_rank, c->S_tot); ALLOC(c->d_perm, c->S_tot);
+    ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
+    ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
+    ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
+#undef ALLOC
+    if (hipMemcpy(c->d_sys, c->hs.data(), n_sys * sizeof(SysDev), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_blk_sys, blk_sys.data(), c->n_blocks * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_cell_sys, cell_sys.data(), c->C_tot * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        free_all(c); delete c; return fail(MFA_E_HIP, "parameter upload failed");
+    }
+    long long t0 = sys[0].t_idx0;
+    if (hipMemcpy(c->B.t_idx, &t0, sizeof(long long), hipMemcpyHostToDevice) != hipSuccess) {
+        free_all(c); delete c; return fail(MFA_E_HIP, "time upload failed");
+    }
+    *out = c;
+    return 0;
+}
+
+int mfa_hip_destroy(mfa_ctx *c) {
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->st);
+    free_all(c);
+    delete c;
+    return 0;
+}
+
+int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *v, const double *a,
+                      const int32_t *a_type) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !r0 || !v || !a_type) return fail(MFA_E_ARG, "set_state: bad argument");
+    const SysDev &d = c->hs[isys];
+    const size_t n = d.n_part;
+    for (size_t i = 0; i < n; ++i)
+        if (a_type[i] < 1 || a_type[i] > 4) return fail(MFA_E_ARG, "set_state: a_type outside 1..4");
+    double *dr = nullptr, *dv = nullptr, *da = nullptr;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMalloc((void **)&dr, 3 * n * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&dv, 3 * n * sizeof(double)));
+    if (a) HIPCHK(hipMalloc((void **)&da, 3 * n * sizeof(double)));
+    HIPCHK(hipMemcpy(dr, r0, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dv, v, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    if (a) HIPCHK(hipMemcpy(da, a, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_istage, a_type, n * sizeof(int), hipMemcpyHostToDevice));
+    launch(c->st, k_upload, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
+           (const double *)da, (const int *)c->d_istage);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    (void)hipFree(dr); (void)hipFree(dv); if (da) (void)hipFree(da);
+    c->uploaded[isys] = 1;
+    c->have_state = std::all_of(c->uploaded.begin(), c->uploaded.end(), [](char u) { return u != 0; });
+    // cell list of the new state (init_config.f90:261-272); pending until every
+    // channel has a state, since the rebuild runs over all of them
+    return 0;
+}
+
+static int ensure_cells(mfa_ctx *c) {
+    // A rebuild is pending after set_state; run it (it is a no-op otherwise).
+    enqueue_rebuild(c);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 0);
+    return 0;
+}
+
+int mfa_hip_get_state(mfa_ctx *c, int32_t isys, double *r0, double *v, double *a, double *force) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys) return fail(MFA_E_ARG, "get_state: bad channel");
+    const SysDev &d = c->hs[isys];
+    const size_t n = d.n_part;
+    double *outs[4] = {r0, v, a, force};
+    for (int w = 0; w < 4; ++w) {
+        if (!outs[w]) continue;
+        launch(c->st, k_download, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, c->d_stage, w);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(outs[w], c->d_stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return 0;
+}
+
+int mfa_hip_set_time(mfa_ctx *c, int64_t t_idx) {
+    if (int rc = check_ctx(c, false)) return rc;
+    long long t = t_idx;
+    HIPCHK(hipMemcpyAsync(c->B.t_idx, &t, sizeof(long long), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int mfa_hip_get_time(mfa_ctx *c, int64_t *t_idx) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!t_idx) return fail(MFA_E_ARG, "get_time: null");
+    long long t = 0;
+    HIPCHK(hipMemcpyAsync(&t, c->B.t_idx, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *t_idx = t;
+    return 0;
+}
+
+int mfa_hip_verlet_positions(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    ensure_cells(c);
+    enqueue_vv1(c);
+    enqueue_rebuild(c);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_check_skin(mfa_ctx *c, int32_t *f_skin) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!f_skin) return fail(MFA_E_ARG, "check_skin: null");
+    // verlet_positions already rebuilt the cells when the skin test fired;
+    // report how many rebuilds happened so far.
+    int v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->d_rebuild_count, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *f_skin = v;
+    return 0;
+}
+
+int mfa_hip_fluid_fluid(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    enqueue_rebuild(c);       // pending rebuild after set_state / verlet_positions (gated)
+    enqueue_pair(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_fluid_wall(mfa_ctx *c, int32_t inter_type) {
+    if (int rc = check_ctx(c)) return rc;
+    if (inter_type != 2) return fail(MFA_E_ARG, "fluid_wall: only inter_type 2 (implicit 9-3 walls) is supported");
+    enqueue_bonded<M_WALL>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_wall_wall(mfa_ctx *c, int32_t inter_type, double sigma_w, double *v_wall_wall) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!v_wall_wall) return fail(MFA_E_ARG, "wall_wall: null");
+    if (inter_type != 2) return fail(MFA_E_ARG, "wall_wall: only inter_type 2 is supported");
+    const mfa_system_params &p = c->hp[0];
+    const double q = sigma_w / p.boundary[2];
+    *v_wall_wall = 2.0 * (fabs(p.a_w) * pow(q, 9) - p.a_w * pow(q, 3));
+    return 0;
+}
+
+int mfa_hip_intra_molec(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    enqueue_bonded<M_FENE>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_bending(mfa_ctx *c, int32_t mode) {
+    if (int rc = check_ctx(c)) return rc;
+    if (mode == 0) return 0;
+    if (mode != 1) return fail(MFA_E_ARG, "bending: mode must be 0 or 1");
+    enqueue_bonded<M_BEND>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_bending_melt(mfa_ctx *c, int32_t mode) {
+    if (int rc = check_ctx(c)) return rc;
+    if (mode == 0) return 0;
+    if (mode != 1) return fail(MFA_E_ARG, "bending_melt: mode must be 0 or 1");
+    enqueue_bonded<M_BENDM>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_constant_force(mfa_ctx *c, int32_t case_pois) {
+    if (int rc = check_ctx(c)) return rc;
+    if (case_pois != 1) return fail(MFA_E_ARG, "constant_force: only case 1 (Poiseuille body force) is supported");
+    enqueue_bonded<M_CONST>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_verlet_velocities(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    enqueue_bonded<M_VV2>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n_steps < 0) return fail(MFA_E_ARG, "md_steps: negative step count");
+    ensure_cells(c);
+    if (!use_graph || c->timing) {
+        for (int i = 0; i < n_steps; ++i) enqueue_step(c);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    if (!c->gexec) {
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c);
+        hipError_t e = hipStreamEndCapture(c->st, &c->graph);
+        if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("capture: ") + hipGetErrorString(e)); }
+        e = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("instantiate: ") + hipGetErrorString(e)); }
+    }
+    for (int i = 0; i < n_steps; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    return 0;
+}
+
+int mfa_hip_synchronize(mfa_ctx *c) {
+    if (int rc = check_ctx(c, false)) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int mfa_hip_get_energies(mfa_ctx *c, int32_t isys, mfa_energies *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !out) return fail(MFA_E_ARG, "get_energies: bad argument");
+    hipLaunchKernelGGL(k_reduce, dim3(c->n_sys), dim3(BLK), 0, c->st, c->d_sys, (const double *)c->d_partial,
+                       c->d_energies);
+    HIPCHK(hipGetLastError());
+    double e[NPART];
+    HIPCHK(hipMemcpyAsync(e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const mfa_system_params &p = c->hp[isys];
+    out->v_fluid_fluid = 0.5 * e[C_VFF];
+    out->v_fluid_wall = e[C_VWALL];
+    out->v_intra_molec = p.k_chain * e[C_VINTRA];
+    out->v_bend = e[C_VBEND];
+    out->v_bend_melt = e[C_VBENDM];
+    out->t_fluid = 0.5 * e[C_KE];
+    for (int q = 0; q < 6; ++q) out->virial[q] = 0.5 * e[C_WXX + q];
+    return 0;
+}
+
+int mfa_hip_get_pair_count(mfa_ctx *c, int32_t isys, int64_t *n_pairs) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !n_pairs) return fail(MFA_E_ARG, "get_pair_count: bad argument");
+    hipLaunchKernelGGL(k_reduce, dim3(c->n_sys), dim3(BLK), 0, c->st, c->d_sys, (const double *)c->d_partial,
+                       c->d_energies);
+    double e[NPART];
+    HIPCHK(hipMemcpyAsync(e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *n_pairs = (int64_t)llround(0.5 * e[C_NPAIR]);
+    return 0;
+}
+
+int mfa_hip_get_f_on_heads(mfa_ctx *c, int32_t isys, double *f) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !f) return fail(MFA_E_ARG, "get_f_on_heads: bad argument");
+    const SysDev &d = c->hs[isys];
+    if (d.n_chain == 0) return 0;
+    HIPCHK(hipMemcpyAsync(f, c->d_fheads + 3 * (size_t)d.head_base, 3 * (size_t)d.n_chain * sizeof(double),
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int mfa_hip_get_flags(mfa_ctx *c, mfa_flags *out) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!out) return fail(MFA_E_ARG, "get_flags: null");
+    int f[8], rb = 0;
+    HIPCHK(hipMemcpyAsync(f, c->B.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&rb, c->d_rebuild_count, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    out->nonfinite = f[0];
+    out->out_of_box_z = f[1];
+    out->fene_stretched = f[2];
+    out->rebuilds = rb;
+    return 0;
+}
+
+static int brute(mfa_ctx *c, int isys, double *e_sum, int32_t *count, uint64_t *hash) {
+    const SysDev &d = c->hs[isys];
+    const size_t n = d.n_part;
+    double *de = nullptr; int *dc = nullptr; unsigned long long *dh = nullptr;
+    HIPCHK(hipMalloc((void **)&de, n * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&dc, n * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&dh, n * sizeof(unsigned long long)));
+    long long t = 0;
+    HIPCHK(hipMemcpyAsync(&t, c->B.t_idx, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const mfa_system_params &p = c->hp[isys];
+    double r_2_min = 0.0;
+    if (p.force_switch_on) {
+        double r_time = (double)t * p.dt;
+        if (r_time < p.r_2_min_time) r_2_min = (1.0 - r_time / p.r_2_min_time) * p.r_2_min_init;
+    }
+    launch(c->st, k_brute, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, r_2_min, de, dc, dh);
+    HIPCHK(hipGetLastError());
+    std::vector<double> he(n);
+    HIPCHK(hipMemcpyAsync(he.data(), de, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (count) HIPCHK(hipMemcpyAsync(count, dc, n * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    if (hash) HIPCHK(hipMemcpyAsync(hash, dh, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += he[i];
+    if (e_sum) *e_sum = s;
+    (void)hipFree(de); (void)hipFree(dc); (void)hipFree(dh);
+    return 0;
+}
+
+int mfa_hip_check_fluid_fluid(mfa_ctx *c, int32_t isys, double *v) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !v) return fail(MFA_E_ARG, "check_fluid_fluid: bad argument");
+    return brute(c, isys, v, nullptr, nullptr);
+}
+
+int mfa_hip_pair_stats(mfa_ctx *c, int32_t isys, int32_t *count, uint64_t *hash) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !count || !hash) return fail(MFA_E_ARG, "pair_stats: bad argument");
+    return brute(c, isys, nullptr, count, hash);
+}
+
+int mfa_hip_get_cells(mfa_ctx *c, int32_t isys, int32_t *n_cells, int32_t *cell_start, int32_t *cell_count,
+                      int32_t *cell_beads) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys) return fail(MFA_E_ARG, "get_cells: bad channel");
+    ensure_cells(c);
+    const SysDev &d = c->hs[isys];
+    if (n_cells) for (int k = 0; k < 3; ++k) n_cells[k] = d.nc[k];
+    std::vector<int2> rg(d.n_cells_tot);
+    HIPCHK(hipMemcpyAsync(rg.data(), c->d_crange + d.cell_base, d.n_cells_tot * sizeof(int2), hipMemcpyDeviceToHost, c->st));
+    int par = 0;
+    HIPCHK(hipMemcpyAsync(&par, c->B.parity, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<double4> pos(d.n_part);
+    HIPCHK(hipMemcpy(pos.data(), c->B.pos[par] + d.slot_base, d.n_part * sizeof(double4), hipMemcpyDeviceToHost));
+    for (int q = 0; q < d.n_cells_tot; ++q) {
+        if (cell_start) cell_start[q] = rg[q].x - d.slot_base;
+        if (cell_count) cell_count[q] = rg[q].y - rg[q].x;
+    }
+    if (cell_beads)
+        for (int i = 0; i < d.n_part; ++i) {
+            uint64_t w;
+            memcpy(&w, &pos[i].w, 8);
+            cell_beads[i] = (int)(w >> 3) - d.part_base;
+        }
+    return 0;
+}
+
+int mfa_hip_profiles_reset(mfa_ctx *c, int32_t n_bins) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (n_bins < 1 || n_bins > 100000) return fail(MFA_E_ARG, "profiles_reset: n_bins out of range");
+    HIPCHK(hipStreamSynchronize(c->st));
+    auto F = [](void *p) { if (p) (void)hipFree(p); };
+    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
+    c->d_pcnt = nullptr; c->d_pvs = c->d_pv2s = nullptr; c->d_pdens = c->d_pvp = c->d_pvp2 = nullptr;
+    const size_t nc = (size_t)c->n_sys * 2 * n_bins, nv = nc * 3;
+    if (int rc = dalloc(&c->d_pcnt, nc)) return rc;
+    if (int rc = dalloc(&c->d_pvs, nv)) return rc;
+    if (int rc = dalloc(&c->d_pv2s, nv)) return rc;
+    if (int rc = dalloc(&c->d_pdens, nc)) return rc;
+    if (int rc = dalloc(&c->d_pvp, nv)) return rc;
+    if (int rc = dalloc(&c->d_pvp2, nv)) return rc;
+    c->prof_nb = n_bins;
+    c->prof_nobs = 0;
+    return 0;
+}
+
+int mfa_hip_profiles_accumulate(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    if (c->prof_nb < 1) return fail(MFA_E_STATE, "profiles_accumulate: call profiles_reset first");
+    launch(c->st, k_prof_bin, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->prof_nb, c->d_pcnt, c->d_pvs, c->d_pv2s);
+    const int nc = c->n_sys * 2 * c->prof_nb, nv = nc * 3;
+    launch(c->st, k_prof_fold, (nv + BLK - 1) / BLK, nc, nv, c->d_pcnt, c->d_pvs, c->d_pv2s, c->d_pdens, c->d_pvp,
+           c->d_pvp2);
+    HIPCHK(hipGetLastError());
+    c->prof_nobs += 1;
+    return 0;
+}
+
+int mfa_hip_profiles_get(mfa_ctx *c, int32_t isys, int32_t *n_obs, double *dens_z, double *v_prof, double *v_prof_2) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (isys < 0 || isys >= c->n_sys || c->prof_nb < 1) return fail(MFA_E_ARG, "profiles_get: bad argument");
+    const size_t nb = c->prof_nb;
+    if (n_obs) *n_obs = c->prof_nobs;
+    // device layout: dens[s][sp][bin], v[s][sp][d][bin]; reference layout:
+    // dens_z(bin, sp), v_prof(bin, d, sp)
+    std::vector<double> hd(2 * nb), hv(6 * nb), hv2(6 * nb);
+    HIPCHK(hipMemcpyAsync(hd.data(), c->d_pdens + (size_t)isys * 2 * nb, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hv.data(), c->d_pvp + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hv2.data(), c->d_pvp2 + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (dens_z) memcpy(dens_z, hd.data(), 2 * nb * sizeof(double));
+    for (int sp = 0; sp < 2; ++sp)
+        for (int d = 0; d < 3; ++d)
+            for (size_t b = 0; b < nb; ++b) {
+                const size_t src = ((size_t)sp * 3 + d) * nb + b;
+                const size_t dst = b + nb * (d + 3 * sp);
+                if (v_prof) v_prof[dst] = hv[src];
+                if (v_prof_2) v_prof_2[dst] = hv2[src];
+            }
+    return 0;
+}
+
+int mfa_hip_set_timing(mfa_ctx *c, int32_t on) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (int rc = collect_timing(c)) return rc;
+    c->timing = on != 0;
+    for (int k = 0; k < K_NK; ++k) { c->k_ms[k] = 0.0; c->k_n[k] = 0; }
+    return 0;
+}
+
+int mfa_hip_get_kernel_time(mfa_ctx *c, const char *kernel, double *total_ms, int64_t *launches) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!kernel || !total_ms || !launches) return fail(MFA_E_ARG, "get_kernel_time: null");
+    if (int rc = collect_timing(c)) return rc;
+    for (int k = 0; k < K_NK; ++k)
+        if (!strcmp(kernel, kNames[k])) { *total_ms = c->k_ms[k]; *launches = c->k_n[k]; return 0; }
+    return fail(MFA_E_ARG, std::string("unknown kernel ") + kernel);
+}
+
+int mfa_hip_set_virial(mfa_ctx *c, int32_t on) {
+    if (int rc = check_ctx(c, false)) return rc;
+    c->virial = on != 0;
+    invalidate_graph(c);
+    return 0;
+}
+
+}  // extern "C"

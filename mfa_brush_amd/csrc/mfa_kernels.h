@@ -1,0 +1,171 @@
+// mfa_kernels.h -- device-side data and arithmetic shared by the gfx950 kernels.
+//
+// Layout in HBM (one context, possibly many channels):
+//   slots   : every channel owns a 256-aligned slot range; within it the
+//             beads are kept sorted by cell (counting sort), ascending
+//             reference index inside a cell.  Padding slots are never in a
+//             cell.
+//   pos     : double4 (x, y, z, w) with w carrying the bits of
+//             (global bead index << 3 | type), so one 32-byte load gives a
+//             bead's position, identity and type.
+//   vel/acc/frc : double4 (x, y, z, unused).
+//   posf/velf   : float4 copies for the mixed path; posf holds the position
+//             relative to the origin of the bead's build-time cell.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfa {
+
+constexpr int BLK = 256;          // threads per block and slot-range alignment
+constexpr int NPART = 16;         // partial-sum columns per block
+enum Col { C_VFF = 0, C_VWALL, C_VINTRA, C_VBEND, C_VBENDM, C_KE,
+           C_WXX, C_WYY, C_WZZ, C_WXY, C_WXZ, C_WYZ, C_NPAIR, C_NCOL };
+
+struct SysDev {
+    double L[3], invL[3];
+    double l_cell[3], inv_l_cell[3];
+    double epsil[16], sigma_2[16], range_2[16], e_shift[16], inv_range_2[16];
+    float  epsil_f[16], sigma_2f[16], range_2f_lo[16], range_2f_hi[16], e_shift_f[16], inv_rc_f[16];
+    double r_cut_dpd_2, sig, friction, r_2_min_init, r_2_min_time;
+    double sigma_wall[4], a_wall[4], a_w;
+    double k_chain, inv_r_chain_2, k_bend, alpha_eq;
+    double dt, dt_2;
+    double const_force[3], head_vel[2];
+    double mass_type[4], inv_mass_type[4];
+    double skin_half_2;
+    uint64_t seed;
+    int nc[3], n_cells_tot, cell_base;
+    int slot_base, n_part, part_base;
+    int n_mon, n_chain, n_mon_d, n_chain_d, pd, head_base;
+    int thermostat, dpd_cut_off, force_switch_on, bending, bending_melt, poiseuille;
+    int S, skin_mode;
+    int block_base, n_blocks;
+    int shift;                    // slot_base - (beads of all earlier channels)
+};
+
+// Double-buffered bead arrays: a rebuild permutes into the other half and a
+// one-thread kernel flips `parity`, so no copy-back pass is needed and the
+// host never has to know whether the device decided to rebuild.
+struct Bufs {
+    double4 *pos[2], *vel[2], *acc[2], *posb[2];
+    float4 *posf[2], *velf[2];
+    int *cell_of[2];
+    double4 *frc;
+    int *where;
+    int *parity;
+    int *rebuild;
+    int *flags;
+    long long *t_idx;
+};
+
+#define MFA_CUR(B) const int par = *(B).parity; (void)par
+
+__device__ __forceinline__ uint64_t wbits(double w) { return (uint64_t)__double_as_longlong(w); }
+__device__ __forceinline__ int wtype(double w) { return (int)(wbits(w) & 7ull); }
+__device__ __forceinline__ int worig(double w) { return (int)(wbits(w) >> 3); }
+__device__ __forceinline__ double mkw(int orig, int type) {
+    return __longlong_as_double((long long)(((uint64_t)orig << 3) | (uint64_t)type));
+}
+__device__ __forceinline__ int fwtype(float w) { return (int)(__float_as_uint(w) & 7u); }
+__device__ __forceinline__ int fworig(float w) { return (int)(__float_as_uint(w) >> 3); }
+__device__ __forceinline__ float mkwf(int orig, int type) { return __uint_as_float(((uint32_t)orig << 3) | (uint32_t)type); }
+
+// ---------------------------------------------------------------------------
+// IEEE-exact pieces.  The pair decision and the integrator are written with
+// FP contraction off so they compute what the reference's x86-64 build
+// computes (no FMA): minimum image d - L*int(2 d/L) (fluid_fluid.f90:111-112),
+// r^2 = (dx^2+dy^2)+dz^2 (:117).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mic(double d, double L, double invL) {
+#pragma clang fp contract(off)
+    return d - L * (double)(int)(2.0 * d * invL);
+}
+__device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Philox-4x32-10 (Salmon et al., SC'11)
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+// Box-Muller normal from two 32-bit words with IEEE +,-,*,/,sqrt only (FP32,
+// correctly rounded divide and square root, no contraction), so the value is
+// the same bits on the host and on the device.
+__device__ __forceinline__ float gauss_from_bits(uint32_t x0, uint32_t x1) {
+#pragma clang fp contract(off)
+    float u1 = (float)((x0 >> 8) + 1u) * 5.9604644775390625e-08f;
+    uint32_t b = __float_as_uint(u1);
+    int e = (int)(b >> 23) - 127;
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    float s = __fdiv_rn(m - 1.0f, m + 1.0f);
+    float s2 = s * s;
+    float p = 0.222222222f;
+    p = p * s2 + 0.285714286f;
+    p = p * s2 + 0.4f;
+    p = p * s2 + 0.666666667f;
+    p = p * s2 + 2.0f;
+    float lnu = (float)e * 0.693147182f + s * p;
+    float rad = __fsqrt_rn(-2.0f * lnu);
+    uint32_t q = x1 >> 8;
+    uint32_t k = (q + (1u << 21)) >> 22;
+    float f = (float)((int32_t)q - (int32_t)(k << 22)) * 2.38418579e-07f;
+    float a = f * 1.57079633f;
+    float a2 = a * a;
+    float c = 2.48015873e-05f;
+    c = c * a2 - 1.38888889e-03f;
+    c = c * a2 + 4.16666667e-02f;
+    c = c * a2 - 0.5f;
+    c = c * a2 + 1.0f;
+    float sn = 2.75573192e-06f;
+    sn = sn * a2 - 1.98412698e-04f;
+    sn = sn * a2 + 8.33333333e-03f;
+    sn = sn * a2 - 1.66666667e-01f;
+    sn = sn * a2 + 1.0f;
+    sn = sn * a;
+    uint32_t kk = k & 3u;
+    float cs = kk == 0u ? c : (kk == 1u ? -sn : (kk == 2u ? -c : sn));
+    return rad * cs;
+}
+
+// The DPD noise of the unordered pair {i, j} at time index t: one normal,
+// identical for (i,j) and (j,i), so the random pair force is exactly
+// antisymmetric although each bead evaluates the pair itself.
+__device__ __forceinline__ float pair_gauss(uint64_t seed, uint64_t t, int i, int j) {
+    uint32_t c[4];
+    c[0] = (uint32_t)(i < j ? i : j);
+    c[1] = (uint32_t)(i < j ? j : i);
+    c[2] = (uint32_t)t;
+    c[3] = (uint32_t)(t >> 32);
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return gauss_from_bits(c[0], c[1]);
+}
+
+__device__ __forceinline__ double ipow(double x, int n) {
+    double r = 1.0;
+    while (n) { if (n & 1) r *= x; x *= x; n >>= 1; }
+    return r;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Cell of a position (get_cell, cell_list.f90:186-213), clamped into the grid.
+__device__ __forceinline__ int cell_of_pos(const SysDev &P, double x, double y, double z) {
+    int cx = clampi((int)(x * P.inv_l_cell[0]), 0, P.nc[0] - 1);
+    int cy = clampi((int)(y * P.inv_l_cell[1]), 0, P.nc[1] - 1);
+    int cz = clampi((int)(z * P.inv_l_cell[2]), 0, P.nc[2] - 1);
+    return P.cell_base + cx + P.nc[0] * (cy + P.nc[1] * cz);
+}
+
+}  // namespace mfa

@@ -1,0 +1,571 @@
+// mfa_step.h -- per-bead kernels of the step: integrator halves, bonded
+// forces, walls, body force, cell-list rebuild (counting sort), observables.
+#pragma once
+#include "mfa_kernels.h"
+#include "mfa_pair.h"
+
+namespace mfa {
+
+
+// ---------------------------------------------------------------------------
+// verlet_positions, verlet_positions.f90:150-209: r += dt v + 0.5 dt^2 a,
+// single wrap in x,y, v += 0.5 dt a; plus the skin test of check_skin.f90:11-24
+// (displacement since the last build > skin/2 -> rebuild).
+// ---------------------------------------------------------------------------
+template <bool MIXED>
+__global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B) {
+    MFA_CUR(B);
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (threadIdx.x == 0 && P.skin_mode == 0) *B.rebuild = 1;
+    if (k - P.slot_base >= P.n_part) return;
+    double4 r = B.pos[par][k];
+    double4 v = B.vel[par][k];
+    const double4 a = B.acc[par][k];
+    {
+#pragma clang fp contract(off)
+        const double h = 0.5 * P.dt_2;
+        r.x = r.x + P.dt * v.x + h * a.x;
+        r.y = r.y + P.dt * v.y + h * a.y;
+        r.z = r.z + P.dt * v.z + h * a.z;
+        if (r.x > P.L[0]) r.x = r.x - P.L[0]; else if (r.x < 0.0) r.x = r.x + P.L[0];
+        if (r.y > P.L[1]) r.y = r.y - P.L[1]; else if (r.y < 0.0) r.y = r.y + P.L[1];
+        const double hd = 0.5 * P.dt;
+        v.x = v.x + hd * a.x;
+        v.y = v.y + hd * a.y;
+        v.z = v.z + hd * a.z;
+    }
+    B.pos[par][k] = r;
+    B.vel[par][k] = v;
+    if (!isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z)) atomicAdd(&B.flags[0], 1);
+    if (r.z < 0.0 || r.z > P.L[2]) atomicAdd(&B.flags[1], 1);
+    if (P.skin_mode) {
+        const double4 b = B.posb[par][k];
+        double dx = r.x - b.x, dy = r.y - b.y, dz = r.z - b.z;
+        dx -= P.L[0] * rint(dx * P.invL[0]);
+        dy -= P.L[1] * rint(dy * P.invL[1]);
+        if (!(dx * dx + dy * dy + dz * dz <= P.skin_half_2)) atomicOr(B.rebuild, 1);
+    }
+    if (MIXED) {
+        const int c = B.cell_of[par][k] - P.cell_base;
+        const int nx = P.nc[0], ny = P.nc[1];
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
+        if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
+        if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
+        float4 pf = B.posf[par][k];
+        pf.x = (float)rx; pf.y = (float)ry; pf.z = (float)rz;
+        B.posf[par][k] = pf;
+        B.velf[par][k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bonded terms
+// ---------------------------------------------------------------------------
+struct BondOut { double fx, fy, fz, pot; };
+
+// FENE bond (a, b=a+1), intra_molec.f90:21-58: returns force_loc (the force
+// the reference adds to b and subtracts from a) and -0.5 ln(1-q).
+__device__ __forceinline__ BondOut fene(const SysDev &P, const double4 &ra, const double4 &rb, int *stretched) {
+    BondOut o;
+    double dx = mic(ra.x - rb.x, P.L[0], P.invL[0]);
+    double dy = mic(ra.y - rb.y, P.L[1], P.invL[1]);
+    double dz = ra.z - rb.z;
+    double r_2 = r2_exact(dx, dy, dz);
+    double q = r_2 * P.inv_r_chain_2;
+    const double clamp = (double)0.98f;
+    if (q > clamp) { *stretched = 1; q = clamp; }
+    o.pot = -0.5 * log(1.0 - q);
+    double rd = P.k_chain / (1.0 - q) * P.inv_r_chain_2;
+    o.fx = rd * dx; o.fy = rd * dy; o.fz = rd * dz;
+    return o;
+}
+
+struct BendOut { double fp[3], fn[3], v; };
+
+// Harmonic-angle bending centred on c, bending.f90:30-109 (including the
+// small-angle fallback :59-92 and the cos >= 1 guard :93-98).
+__device__ BendOut bend_angle(const SysDev &P, const double4 &rm, const double4 &rc, const double4 &rpn) {
+    BendOut o;
+    double rp[3], rn[3];
+    rp[0] = mic(rc.x - rm.x, P.L[0], P.invL[0]);
+    rp[1] = mic(rc.y - rm.y, P.L[1], P.invL[1]);
+    rp[2] = rc.z - rm.z;
+    rn[0] = mic(rpn.x - rc.x, P.L[0], P.invL[0]);
+    rn[1] = mic(rpn.y - rc.y, P.L[1], P.invL[1]);
+    rn[2] = rpn.z - rc.z;
+    double dot = 0.0, rp2 = 0.0, rn2 = 0.0;
+    for (int d = 0; d < 3; ++d) { dot += rn[d] * rp[d]; rp2 += rp[d] * rp[d]; rn2 += rn[d] * rn[d]; }
+    double cosa = dot / sqrt(rp2 * rn2);
+    double dn[3], dp[3], dn2 = 0.0, dp2 = 0.0;
+    for (int d = 0; d < 3; ++d) { dn[d] = rp[d] * rn2 - rn[d] * dot; dp[d] = -rn[d] * rp2 + rp[d] * dot; }
+    for (int d = 0; d < 3; ++d) { dp2 += dp[d] * dp[d]; dn2 += dn[d] * dn[d]; }
+    const double small = (double)0.000001f;
+    if (dn2 < small) {
+        double s1 = sqrt(rp2), s2 = sqrt(rn2);
+        for (int d = 0; d < 3; ++d) dn[d] = rp[d] / s1 - rn[d] / s2;
+        double nrm = sqrt(dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2]);
+        for (int d = 0; d < 3; ++d) dn[d] = nrm != 0.0 ? dn[d] / nrm / s2 : 0.0;
+    } else {
+        double sc = sqrt(dn2 * rn2);
+        for (int d = 0; d < 3; ++d) dn[d] = dn[d] / sc;
+    }
+    if (dp2 < small) {
+        double s1 = sqrt(rp2), s2 = sqrt(rn2);
+        for (int d = 0; d < 3; ++d) dp[d] = rp[d] / s1 - rn[d] / s2;
+        double nrm = sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+        for (int d = 0; d < 3; ++d) dp[d] = nrm != 0.0 ? dp[d] / nrm / s1 : 0.0;
+    } else {
+        double sc = sqrt(dp2 * rp2);
+        for (int d = 0; d < 3; ++d) dp[d] = dp[d] / sc;
+    }
+    double dal = cosa >= 1.0 ? 0.0 : acos(cosa) - P.alpha_eq;
+    double fm = P.k_bend * dal;
+    o.v = 0.5 * fm * dal;
+    for (int d = 0; d < 3; ++d) { o.fp[d] = fm * dp[d]; o.fn[d] = fm * dn[d]; }
+    return o;
+}
+
+enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64 };
+
+__device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
+    return B.pos[par][B.where[gorig]];
+}
+
+// Bending contributions to the bead at chain position m (0-based) of a chain
+// of nm beads starting at global index g0, in the reference's loop order
+// (centres ascending): +fn of centre m-1, -fp-fn of centre m, +fp of m+1.
+__device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, int par, int g0, int nm, int m,
+                                          const double4 &ri, double3 &F, double &ev) {
+    double4 r[5];
+    for (int q = -2; q <= 2; ++q) {
+        int mm = m + q;
+        r[q + 2] = (q == 0) ? ri : ((mm >= 0 && mm < nm) ? bead_at(B, par, g0 + mm) : ri);
+    }
+    if (m - 1 >= 1 && m - 1 <= nm - 2) {
+        BendOut o = bend_angle(P, r[0], r[1], r[2]);
+        F.x += o.fn[0]; F.y += o.fn[1]; F.z += o.fn[2];
+    }
+    if (m >= 1 && m <= nm - 2) {
+        BendOut o = bend_angle(P, r[1], r[2], r[3]);
+        F.x = F.x - o.fp[0] - o.fn[0];
+        F.y = F.y - o.fp[1] - o.fn[1];
+        F.z = F.z - o.fp[2] - o.fn[2];
+        ev += o.v;
+    }
+    if (m + 1 >= 1 && m + 1 <= nm - 2) {
+        BendOut o = bend_angle(P, r[2], r[3], r[4]);
+        F.x += o.fp[0]; F.y += o.fp[1]; F.z += o.fp[2];
+    }
+}
+
+// Everything after the pair forces, per bead, in md_main.f90's order:
+// fluid_wall(2) :132, intra_molec :142, bending(1) :144, bending_melt(1) :147,
+// constant_force(1) :170, verlet_velocities :185.  MODE selects the terms
+// (one term for the single-routine entry points, all for the fused step).
+template <int MODE, bool MIXED>
+__global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                               double *__restrict__ partial, double *__restrict__ f_heads) {
+    __shared__ double red[BLK];
+    MFA_CUR(B);
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    const bool active = (k - P.slot_base) < P.n_part;
+    constexpr bool FUSED = (MODE & M_FUSED) != 0;
+    double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
+    if (active) {
+        const double4 ri = B.pos[par][k];
+        const int ti = wtype(ri.w);
+        const int g = worig(ri.w);
+        const int o = g - P.part_base;
+        double4 f4 = B.frc[k];
+        double3 F = make_double3(f4.x, f4.y, f4.z);
+        const bool brush = o < P.pd;
+        const int nm = brush ? P.n_mon : P.n_mon_d;
+        const int ol = brush ? o : o - P.pd;
+        const int m = nm > 0 ? ol % nm : 0;
+        const int g0 = g - m;
+        if (MODE & (M_WALL | M_FUSED)) {
+            // fluid_wall.f90:93-121; energy with the global a_w (Q9)
+            const double sw = P.sigma_wall[ti - 1], aw = P.a_wall[ti - 1];
+            double inv_z = 1.0 / ri.z;
+            double rd = sw * inv_z;
+            e_wall += fabs(aw) * ipow(rd, 9) - P.a_w * ipow(rd, 3);
+            F.z += 9.0 * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
+            F.z -= 3.0 * aw * ipow(sw, 3) * ipow(inv_z, 4);
+            inv_z = 1.0 / (P.L[2] - ri.z);
+            rd = sw * inv_z;
+            e_wall += fabs(aw) * ipow(rd, 9) - P.a_w * ipow(rd, 3);
+            F.z -= 9.0 * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
+            F.z += 3.0 * aw * ipow(sw, 3) * ipow(inv_z, 4);
+        }
+        if (MODE & (M_FENE | M_FUSED)) {
+            int str = 0;
+            if (m > 0) {
+                BondOut b = fene(P, bead_at(B, par, g - 1), ri, &str);
+                F.x += b.fx; F.y += b.fy; F.z += b.fz;
+            }
+            if (m < nm - 1) {
+                BondOut b = fene(P, ri, bead_at(B, par, g + 1), &str);
+                F.x -= b.fx; F.y -= b.fy; F.z -= b.fz;
+                e_fene += b.pot;
+            }
+            if (str) atomicAdd(&B.flags[2], 1);
+        }
+        if ((MODE & M_BEND) || (FUSED && P.bending)) {
+            if (brush) bend_bead(P, B, par, g0, nm, m, ri, F, e_bend);
+        }
+        if ((MODE & M_BENDM) || (FUSED && P.bending_melt)) {
+            if (!brush) bend_bead(P, B, par, g0, nm, m, ri, F, e_bendm);
+        }
+        if ((MODE & M_CONST) || (FUSED && P.poiseuille)) {
+            if (ti != 1) {
+                const double ms = P.mass_type[ti - 1];
+                F.x += ms * P.const_force[0];
+                F.y += ms * P.const_force[1];
+                F.z += ms * P.const_force[2];
+            }
+        }
+        if (MODE & (M_VV2 | M_FUSED)) {
+            const bool head = brush && m == 0;
+            if (head) {
+                double *fh = f_heads + 3 * (size_t)(P.head_base + ol / P.n_mon);
+                fh[0] = F.x; fh[1] = F.y; fh[2] = F.z;
+            }
+            double4 v = B.vel[par][k];
+            double4 a;
+            const double im = P.inv_mass_type[ti - 1];
+            {
+#pragma clang fp contract(off)
+                a.x = F.x * im; a.y = F.y * im; a.z = F.z * im; a.w = 0.0;
+                const double hd = 0.5 * P.dt;
+                v.x = v.x + hd * a.x; v.y = v.y + hd * a.y; v.z = v.z + hd * a.z;
+            }
+            if (head) {
+                if (P.poiseuille) { v.x = 0.0; v.y = 0.0; }
+                else {
+                    const double sg = (ol / P.n_mon) < P.n_chain / 2 ? 1.0 : -1.0;
+                    v.x = sg * P.head_vel[0]; v.y = sg * P.head_vel[1];
+                }
+                v.z = 0.0;
+                a.x = a.y = a.z = 0.0;
+                F.x = F.y = F.z = 0.0;
+            }
+            ke = P.mass_type[ti - 1] * (v.x * v.x + v.y * v.y + v.z * v.z);
+            B.vel[par][k] = v;
+            B.acc[par][k] = a;
+            if (MIXED) B.velf[par][k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+        }
+        f4.x = F.x; f4.y = F.y; f4.z = F.z;
+        B.frc[k] = f4;
+    }
+    double *out = partial + (size_t)blockIdx.x * NPART;
+    if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
+    if (MODE & (M_FENE | M_FUSED)) { double t = block_sum(e_fene, red); if (threadIdx.x == 0) out[C_VINTRA] = t; }
+    if ((MODE & M_BEND) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
+    if ((MODE & M_BENDM) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
+    if (MODE & (M_VV2 | M_FUSED)) {
+        double t = block_sum(ke, red);
+        if (threadIdx.x == 0) out[C_KE] = t;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
+    }
+}
+
+// Per-channel reduction of the block partials in fixed order.
+__global__ __launch_bounds__(BLK) void k_reduce(const SysDev *__restrict__ sys, const double *__restrict__ partial,
+                                               double *__restrict__ energies) {
+    __shared__ double red[BLK];
+    const SysDev &P = sys[blockIdx.x];
+    for (int col = 0; col < C_NCOL; ++col) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < P.n_blocks; b += BLK) acc += partial[(size_t)(P.block_base + b) * NPART + col];
+        double t = block_sum(acc, red);
+        if (threadIdx.x == 0) energies[blockIdx.x * NPART + col] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell-list rebuild: counting sort of the beads by cell (replaces the
+// per-bead relinking update_part_cell, cell_list.f90:215-268, and
+// neigh_list, :56-106).  Every kernel is gated on *rebuild.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_set_flag(int *flag, int v) { *flag = v; }
+
+__global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                            int *__restrict__ cell_count, int *__restrict__ bin_cell,
+                                            int *__restrict__ bin_rank, int n_slots) {
+    if (!*B.rebuild) return;
+    MFA_CUR(B);
+    for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
+        const SysDev &P = sys[blk_sys[k / BLK]];
+        if (k - P.slot_base >= P.n_part) continue;
+        const double4 r = B.pos[par][k];
+        const int c = cell_of_pos(P, r.x, r.y, r.z);
+        bin_cell[k] = c;
+        bin_rank[k] = atomicAdd(&cell_count[c], 1);
+    }
+}
+
+// exclusive scan, 1024 cells per block (4 per thread)
+__global__ __launch_bounds__(BLK) void k_scan1(const int *__restrict__ rebuild, const int *__restrict__ cnt, int n,
+                                              int *__restrict__ pre, int *__restrict__ bsum) {
+    if (!*rebuild) return;
+    __shared__ int sh[BLK];
+    const int base = blockIdx.x * 4 * BLK + 4 * threadIdx.x;
+    int v[4], t = 0;
+    for (int i = 0; i < 4; ++i) { v[i] = (base + i < n) ? cnt[base + i] : 0; t += v[i]; }
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < BLK; off <<= 1) {
+        int x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int ex = sh[threadIdx.x] - t;
+    for (int i = 0; i < 4; ++i) { if (base + i < n) pre[base + i] = ex; ex += v[i]; }
+    if (threadIdx.x == BLK - 1) bsum[blockIdx.x] = sh[BLK - 1];
+}
+
+__global__ __launch_bounds__(BLK) void k_scan2(const int *__restrict__ rebuild, int *__restrict__ bsum, int nb) {
+    if (!*rebuild) return;
+    __shared__ int sh[BLK];
+    int carry = 0;
+    for (int base = 0; base < nb; base += BLK) {
+        const int i = base + threadIdx.x;
+        const int t = i < nb ? bsum[i] : 0;
+        sh[threadIdx.x] = t;
+        __syncthreads();
+        for (int off = 1; off < BLK; off <<= 1) {
+            int x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += x;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - t;
+        carry += sh[BLK - 1];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_scan3(const SysDev *__restrict__ sys, const int *__restrict__ rebuild,
+                                              int *__restrict__ cnt, const int *__restrict__ pre,
+                                              const int *__restrict__ bsum, const int *__restrict__ cell_sys,
+                                              int2 *__restrict__ crange, int n) {
+    if (!*rebuild) return;
+    for (int c = blockIdx.x * BLK + threadIdx.x; c < n; c += gridDim.x * BLK) {
+        const int st = pre[c] + bsum[c / (4 * BLK)] + sys[cell_sys[c]].shift;
+        crange[c] = make_int2(st, st + cnt[c]);
+        cnt[c] = 0;
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+                                                const int *__restrict__ rebuild, const int *__restrict__ bin_cell,
+                                                const int *__restrict__ bin_rank, const int2 *__restrict__ crange,
+                                                int *__restrict__ perm, int n_slots) {
+    if (!*rebuild) return;
+    for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
+        const SysDev &P = sys[blk_sys[k / BLK]];
+        if (k - P.slot_base >= P.n_part) continue;
+        perm[crange[bin_cell[k]].x + bin_rank[k]] = k;
+    }
+}
+
+// Order each cell by reference bead index (atomics made the rank order
+// arbitrary); insertion sort, cells hold a few tens of beads.
+__global__ __launch_bounds__(BLK) void k_cellsort(Bufs B, const int2 *__restrict__ crange,
+                                                 int *__restrict__ perm, int n_cells) {
+    if (!*B.rebuild) return;
+    MFA_CUR(B);
+    for (int c = blockIdx.x * BLK + threadIdx.x; c < n_cells; c += gridDim.x * BLK) {
+        const int2 rg = crange[c];
+        for (int i = rg.x + 1; i < rg.y; ++i) {
+            const int v = perm[i];
+            const int key = worig(B.pos[par][v].w);
+            int j = i - 1;
+            while (j >= rg.x && worig(B.pos[par][perm[j]].w) > key) { perm[j + 1] = perm[j]; --j; }
+            perm[j + 1] = v;
+        }
+    }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                                const int *__restrict__ perm, const int *__restrict__ bin_cell,
+                                                int n_slots) {
+    if (!*B.rebuild) return;
+    MFA_CUR(B);
+    const int nw = par ^ 1;
+    for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
+        const SysDev &P = sys[blk_sys[m / BLK]];
+        if (m - P.slot_base >= P.n_part) continue;
+        const int old = perm[m];
+        const double4 r = B.pos[par][old];
+        const double4 v = B.vel[par][old];
+        B.pos[nw][m] = r;
+        B.vel[nw][m] = v;
+        B.acc[nw][m] = B.acc[par][old];
+        B.posb[nw][m] = r;
+        const int cg = bin_cell[old];
+        B.cell_of[nw][m] = cg;
+        B.where[worig(r.w)] = m;
+        if (MIXED) {
+            const int c = cg - P.cell_base;
+            const int nx = P.nc[0], ny = P.nc[1];
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
+            if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
+            if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
+            B.posf[nw][m] = make_float4((float)rx, (float)ry, (float)rz, mkwf(worig(r.w), wtype(r.w)));
+            B.velf[nw][m] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+        }
+    }
+}
+
+__global__ void k_flip(Bufs B, int *rebuild_count) {
+    if (!*B.rebuild) return;
+    *B.parity ^= 1;
+    *rebuild_count += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Host <-> device in reference order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_upload(const SysDev *__restrict__ sys, int s, Bufs B, const double *__restrict__ r0,
+                                               const double *__restrict__ v, const double *__restrict__ a,
+                                               const int *__restrict__ type) {
+    MFA_CUR(B);
+    const SysDev &P = sys[s];
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= P.n_part) return;
+    const int k = P.slot_base + i;
+    const int g = P.part_base + i;
+    B.pos[par][k] = make_double4(r0[3 * i], r0[3 * i + 1], r0[3 * i + 2], mkw(g, type[i]));
+    B.vel[par][k] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
+    B.acc[par][k] = a ? make_double4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0) : make_double4(0.0, 0.0, 0.0, 0.0);
+    B.frc[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+    B.cell_of[par][k] = -1;
+    B.where[g] = k;
+}
+
+__global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys, int s, Bufs B, double *__restrict__ out,
+                                                 int which) {
+    MFA_CUR(B);
+    const SysDev &P = sys[s];
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= P.n_part) return;
+    const int k = B.where[P.part_base + i];
+    double4 q = which == 0 ? B.pos[par][k] : which == 1 ? B.vel[par][k] : which == 2 ? B.acc[par][k] : B.frc[k];
+    out[3 * i] = q.x; out[3 * i + 1] = q.y; out[3 * i + 2] = q.z;
+}
+
+// ---------------------------------------------------------------------------
+// check_fluid_fluid, util.f90:284-330: O(N^2) brute-force LJ energy (and the
+// pair-set fingerprint), one thread per bead i over all j != i.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, int s, Bufs B, double r_2_min,
+                                              double *__restrict__ e_out, int *__restrict__ cnt_out,
+                                              unsigned long long *__restrict__ hash_out) {
+    MFA_CUR(B);
+    __shared__ double4 tile[BLK];
+    const SysDev &P = sys[s];
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    const bool act = i < P.n_part;
+    double4 ri = act ? B.pos[par][B.where[P.part_base + i]] : make_double4(0, 0, 0, 0);
+    const int ti = act ? wtype(ri.w) : 1;
+    double e = 0.0;
+    int cnt = 0;
+    unsigned long long h = 0ull;
+    for (int j0 = 0; j0 < P.n_part; j0 += BLK) {
+        const int jj = j0 + threadIdx.x;
+        tile[threadIdx.x] = jj < P.n_part ? B.pos[par][B.where[P.part_base + jj]] : make_double4(0, 0, 0, 0);
+        __syncthreads();
+        const int jn = min(BLK, P.n_part - j0);
+        if (act) {
+            for (int q = 0; q < jn; ++q) {
+                const int j = j0 + q;
+                if (j == i) continue;
+                const double4 rj = tile[q];
+                const int tj = wtype(rj.w);
+                const int tt = (ti - 1) + 4 * (tj - 1);
+                double dx = mic(ri.x - rj.x, P.L[0], P.invL[0]);
+                double dy = mic(ri.y - rj.y, P.L[1], P.invL[1]);
+                double dz = ri.z - rj.z;
+                double r_2 = r2_exact(dx, dy, dz);
+                if (r_2 < P.range_2[tt]) {
+                    ++cnt;
+                    h += mix64((unsigned long long)j);
+                    if (j > i) {
+                        if (r_2 < r_2_min) r_2 = r_2_min;
+                        double qd = P.sigma_2[tt] / r_2;
+                        double r_6 = qd * qd * qd;
+                        double r_12 = r_6 * r_6;
+                        e += P.epsil[tt] * ((r_12 - r_6) - P.e_shift[tt]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (act) { e_out[i] = e; cnt_out[i] = cnt; hash_out[i] = h; }
+}
+
+// ---------------------------------------------------------------------------
+// Profiles: histo(2) functions.f90:157-213 and velocity_prof(2) :319-365.
+// Per observation the sums go through 64-bit fixed point (order-independent,
+// exact), then one thread per bin adds them to the FP64 running sums.
+// ---------------------------------------------------------------------------
+constexpr double FX_V = 4294967296.0;      // 2^32 for v
+constexpr double FX_V2 = 4294967296.0;     // 2^32 for v^2
+
+__global__ __launch_bounds__(BLK) void k_prof_bin(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                                 int nb, unsigned int *__restrict__ cnt,
+                                                 unsigned long long *__restrict__ vs,
+                                                 unsigned long long *__restrict__ v2s) {
+    MFA_CUR(B);
+    const int s = blk_sys[blockIdx.x];
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k - P.slot_base >= P.n_part) return;
+    const double4 r = B.pos[par][k];
+    const double4 v = B.vel[par][k];
+    const int o = worig(r.w) - P.part_base;
+    const int sp = o < P.pd ? 0 : 1;
+    const double inv_step = 1.0 / (P.L[2] / (double)nb);
+    int z = (int)(r.z * inv_step) + 1;
+    z = z < 1 ? 1 : (z > nb ? nb : z);
+    const size_t base = (size_t)s * 2 * nb + (size_t)sp * nb + (z - 1);
+    atomicAdd(&cnt[base], 1u);
+    const double vv[3] = {v.x, v.y, v.z};
+    for (int d = 0; d < 3; ++d) {
+        const size_t o3 = ((size_t)s * 2 * 3 + (size_t)sp * 3 + d) * nb + (z - 1);
+        atomicAdd(&vs[o3], (unsigned long long)(long long)llrint(vv[d] * FX_V));
+        atomicAdd(&v2s[o3], (unsigned long long)(long long)llrint(vv[d] * vv[d] * FX_V2));
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_prof_fold(int n_cnt, int n_v, unsigned int *__restrict__ cnt,
+                                                  unsigned long long *__restrict__ vs,
+                                                  unsigned long long *__restrict__ v2s, double *__restrict__ dens,
+                                                  double *__restrict__ vp, double *__restrict__ vp2) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i < n_cnt) { dens[i] += (double)cnt[i]; cnt[i] = 0u; }
+    if (i < n_v) {
+        vp[i] += (double)(long long)vs[i] / FX_V;
+        vp2[i] += (double)(long long)v2s[i] / FX_V2;
+        vs[i] = 0ull; v2s[i] = 0ull;
+    }
+}
+
+}  // namespace mfa

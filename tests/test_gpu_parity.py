@@ -1,0 +1,341 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: the interacting-pair set is bit-exact (per-bead partner counts and
+hashes, and the kernel's pair count); FP64 forces agree with the oracle to
+max|dF| <= 1e-11 max|F| (summation order differs: full shell per bead on the
+GPU, half shell + Newton-3 in the reference); energies to 1e-11 relative;
+multi-step trajectories to 1e-9 over 10 steps; the mixed path's forces to
+<= 1e-5 of max|F| with the identical pair set.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mfa_brush_amd import configs
+from mfa_brush_amd.params import Flags
+from mfa_brush_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FTOL = 1e-11
+
+
+def _perturbed_c1(seed=5, **kw):
+    p, t, r0, v = configs.channel_c1(**kw)
+    rng = np.random.default_rng(seed)
+    r0 = r0 + rng.normal(0, 0.05, r0.shape)
+    r0[:, :2] %= 16.0
+    return p, t, r0, v
+
+
+def _close(a, b, tol=FTOL):
+    scale = max(np.abs(b).max(), 1e-30)
+    err = np.abs(a - b).max() / scale
+    assert err <= tol, "max rel err %.3e > %.1e" % (err, tol)
+
+
+def _oracle(oracle_mod, p):
+    return oracle_mod.Oracle(p)
+
+
+def test_try_fixture_golden_on_gpu(try_fixture, oracle_mod):
+    p = try_fixture["p"]
+    with Simulation(p) as s:
+        s.set_state(try_fixture["r0"], try_fixture["v"], try_fixture["a_type"], try_fixture["a"])
+        assert abs(s.check_fluid_fluid() - try_fixture["v_ff"]) < 2e-10
+        pt = p.replace(flags=Flags(**{**p.flags.__dict__, "thermostat": False}))
+    with Simulation(pt) as s:
+        s.set_state(try_fixture["r0"], try_fixture["v"], try_fixture["a_type"], try_fixture["a"])
+        s.fluid_fluid()
+        e = s.energies()["v_fluid_fluid"]
+        assert abs(e - try_fixture["v_ff"]) < 1e-9
+        f = s.get_state()["force"]
+        fo, _ = _oracle(oracle_mod, pt).fluid_fluid(try_fixture["r0"], try_fixture["v"], try_fixture["a_type"], noise=0)
+        _close(f, fo)
+        cnt, h = s.pair_stats()
+        co, ho = _oracle(oracle_mod, pt).pair_stats(try_fixture["r0"], try_fixture["a_type"])
+        assert np.array_equal(cnt, co) and np.array_equal(h, ho)
+        assert s.pair_count() == co.sum() // 2
+
+
+@pytest.mark.parametrize("thermostat,noise,rc_dpd", [(False, 0, None), (True, 0, None), (True, 1, None),
+                                                      (True, 1, 1.0)])
+def test_pair_forces_fp64(oracle_mod, thermostat, noise, rc_dpd):
+    p, t, r0, v = _perturbed_c1(thermostat=thermostat, rc_dpd=rc_dpd)
+    if thermostat and noise == 0:
+        p = p.replace(sig=0.0)                      # dissipative part only
+    step = 17
+    with Simulation(p, t_idx0=step) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+        e = s.energies()["v_fluid_fluid"]
+        np_gpu = s.pair_count()
+    fo, eo = _oracle(oracle_mod, p).fluid_fluid(r0, v, t, step=step, noise=1)
+    _close(f, fo)
+    assert e == pytest.approx(eo, rel=1e-11)
+    cnt, _ = _oracle(oracle_mod, p).pair_stats(r0, t)
+    assert np_gpu == cnt.sum() // 2
+
+
+@pytest.mark.parametrize("l_box,skin", [(1, 0.0), (2, 0.0), (1, 0.4), (2, 0.3)])
+def test_cell_geometries_same_forces(oracle_mod, l_box, skin):
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=skin, flags=Flags(**{**p.flags.__dict__, "l_box": l_box}))
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+    fo, _ = _oracle(oracle_mod, p).fluid_fluid(r0, v, t, step=0, noise=1)
+    _close(f, fo)
+
+
+def _chain_system(bending_melt=False):
+    """Brush (128 x 16) + melt chains (256 x 8) with bending and a body force."""
+    p = configs.channel_params((16.0, 16.0, 20.0), 16, 128, 256, thermostat=False, bending=True, k_bend=5.0,
+                               const_force=(0.01, 0.0, 0.002))
+    p = p.replace(n_mon_d=8, flags=Flags(**{**p.flags.__dict__, "bending_melt": bending_melt}))
+    t, r0, v = configs.build_channel(p, 2.0, 8)
+    rng = np.random.default_rng(2)
+    r0 = r0 + rng.normal(0, 0.04, r0.shape)
+    r0[:, :2] %= 16.0
+    return p, t, r0, v
+
+
+def test_bonded_wall_body_force_routines(oracle_mod):
+    p, t, r0, v = _chain_system(bending_melt=True)
+    o = _oracle(oracle_mod, p)
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f_pair = s.get_state()["force"]
+        s.fluid_wall(2)
+        f1 = s.get_state()["force"]
+        s.intra_molec()
+        f2 = s.get_state()["force"]
+        s.bending(1)
+        f3 = s.get_state()["force"]
+        s.bending_melt(1)
+        f4 = s.get_state()["force"]
+        s.constant_force(1)
+        f5 = s.get_state()["force"]
+        e = s.energies()
+    fw, ew = o.fluid_wall(r0, t)
+    _close(f1 - f_pair, fw, 1e-12)
+    fb, eb, _ = o.intra_molec(r0)
+    _close(f2 - f1, fb, 1e-11)
+    fbe, ebe = o.bending(r0, 0)
+    _close(f3 - f2, fbe, 1e-10)
+    fbm, ebm = o.bending(r0, 1)
+    _close(f4 - f3, fbm, 1e-10)
+    _close(f5 - f4, o.constant_force(t), 1e-14)
+    assert e["v_fluid_wall"] == pytest.approx(ew, rel=1e-12)
+    assert e["v_intra_molec"] == pytest.approx(eb, rel=1e-12)
+    assert e["v_bend"] == pytest.approx(ebe, rel=1e-11)
+    assert e["v_bend_melt"] == pytest.approx(ebm, rel=1e-11)
+
+
+def test_integrator_routines(oracle_mod):
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    o = _oracle(oracle_mod, p)
+    rng = np.random.default_rng(9)
+    a = rng.normal(0, 3.0, r0.shape)
+    heads = np.arange(p.n_chain) * p.n_mon
+    a[heads] = 0.0
+    with Simulation(p) as s:
+        s.set_state(r0, v, t, a)
+        s.verlet_positions()
+        st = s.get_state()
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+        s.verlet_velocities()
+        st2 = s.get_state()
+        ke = s.energies()["t_fluid"]
+        fh = s.f_on_heads()
+    r1, v1 = o.verlet_positions(r0, v, a)
+    assert np.array_equal(st["r0"], r1)                     # same IEEE ops, no contraction
+    assert np.array_equal(st["v"], v1)
+    v2, a2, ke_o, fh_o = o.verlet_velocities(f, st["v"], t)
+    assert np.array_equal(st2["v"], v2) and np.array_equal(st2["a"], a2)
+    assert ke == pytest.approx(ke_o, rel=1e-13)
+    assert np.array_equal(fh, fh_o)
+
+
+@pytest.mark.parametrize("thermostat", [False, True])
+@pytest.mark.parametrize("n_steps", [1, 10])
+def test_fused_steps_vs_oracle(oracle_mod, thermostat, n_steps):
+    p, t, r0, v = _chain_system(bending_melt=False)
+    p = p.replace(flags=Flags(**{**p.flags.__dict__, "thermostat": thermostat}), r_2_min_init=0.0,
+                  r_2_min_time=0.0)
+    o = _oracle(oracle_mod, p)
+    a0 = np.zeros_like(r0)
+    ro, vo, ao, fo, eo, fho = o.md_steps(r0, v, a0, t, 5, n_steps, noise=1)
+    for graph in (False, True):
+        with Simulation(p, t_idx0=5) as s:
+            s.set_state(r0, v, t, a0)
+            s.md_steps(n_steps, graph=graph)
+            st = s.get_state()
+            e = s.energies()
+            assert s.get_time() == 5 + n_steps
+        tol = 1e-12 if n_steps == 1 else 1e-9
+        _close(st["r0"], ro, tol)
+        _close(st["v"], vo, 1e3 * tol)
+        _close(st["force"], fo, 1e3 * tol)
+        for k in ("v_fluid_fluid", "v_intra_molec", "v_fluid_wall", "v_bend", "t_fluid"):
+            assert e[k] == pytest.approx(eo[k], rel=1e3 * tol, abs=1e-9), k
+
+
+def test_routines_equal_fused_step(oracle_mod):
+    p, t, r0, v = _chain_system(bending_melt=False)
+    with Simulation(p) as s1, Simulation(p) as s2:
+        s1.set_state(r0, v, t)
+        s2.set_state(r0, v, t)
+        for _ in range(3):
+            s1.step_by_routines()
+        s2.md_steps(3)
+        a, b = s1.get_state(), s2.get_state()
+        for k in ("r0", "v", "a"):
+            _close(a[k], b[k], 1e-12)
+
+
+def test_determinism_and_graph_replay():
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    outs = []
+    for graph in (False, True, True):
+        with Simulation(p) as s:
+            s.set_state(r0, v, t)
+            s.md_steps(25, graph=graph)
+            outs.append(s.get_state())
+    for o in outs[1:]:
+        for k in ("r0", "v", "a", "force"):
+            assert np.array_equal(o[k], outs[0][k]), k
+
+
+def test_batched_replicas_equal_single_runs():
+    ps = []
+    states = []
+    for k, (n_side, eps_bs) in enumerate([(6, 1.0), (9, 0.7), (5, 1.3)]):
+        p, t, r0, v = configs.channel_scaled(n_side, n_mon=8, lz=14.0, eps_bs=eps_bs, seed=100 + k, skin=0.3)
+        ps.append(p)
+        states.append((t, r0, v))
+    with Simulation(ps) as sb:
+        for i, (t, r0, v) in enumerate(states):
+            sb.set_state(r0, v, t, isys=i)
+        sb.md_steps(20, graph=True)
+        batched = [sb.get_state(i) for i in range(3)]
+        eb = [sb.energies(i) for i in range(3)]
+    for i, (t, r0, v) in enumerate(states):
+        with Simulation(ps[i]) as s:
+            s.set_state(r0, v, t)
+            s.md_steps(20)
+            single = s.get_state()
+            es = s.energies()
+        for k in ("r0", "v", "a"):
+            assert np.array_equal(batched[i][k], single[k]), (i, k)
+        assert eb[i]["v_fluid_fluid"] == es["v_fluid_fluid"]
+
+
+def test_mixed_precision_force_error(oracle_mod):
+    p, t, r0, v = _perturbed_c1(thermostat=False, rc_dpd=None)
+    p = p.replace(skin=0.3)
+    with Simulation(p, precision="mixed") as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+        e = s.energies()["v_fluid_fluid"]
+        npair = s.pair_count()
+    fo, eo = _oracle(oracle_mod, p).fluid_fluid(r0, v, t, noise=0)
+    cnt, _ = _oracle(oracle_mod, p).pair_stats(r0, t)
+    assert npair == cnt.sum() // 2                      # FP64-decided pair set
+    err = np.linalg.norm(f - fo, axis=1).max() / np.linalg.norm(fo, axis=1).max()
+    assert err < 1e-5, err
+    rms = np.sqrt(((f - fo) ** 2).sum(1).mean() / (fo ** 2).sum(1).mean())
+    assert rms < 2e-6, rms
+    assert e == pytest.approx(eo, rel=1e-5)
+
+
+def test_mixed_precision_dpd_statistics():
+    """Thermostat on: the mixed path keeps kT=1 (fluctuation-dissipation)."""
+    p, t, r0, v = configs.channel_c1(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    with Simulation(p, precision="mixed") as s:
+        s.set_state(r0, v, t)
+        s.md_steps(400, graph=True)
+        kes = []
+        for _ in range(20):
+            s.md_steps(20, graph=True)
+            kes.append(s.energies()["t_fluid"])
+        fl = s.flags()
+    n_free = p.n_part - p.n_chain
+    kT = 2.0 * np.mean(kes) / (3 * n_free)
+    assert abs(kT - 1.0) < 0.05, kT
+    assert fl["nonfinite"] == 0
+
+
+def test_pair_set_scaled_c2(oracle_mod):
+    """C2 geometry (n_side 30, ~86k beads): per-bead pair-set fingerprint."""
+    p, t, r0, v = configs.channel_scaled(30)
+    rng = np.random.default_rng(1)
+    r0 = r0 + rng.normal(0, 0.05, r0.shape)
+    r0[:, :2] %= p.boundary[0]
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        cnt, h = s.pair_stats()
+        s.fluid_fluid()
+        npair = s.pair_count()
+    co, ho = _oracle(oracle_mod, p).pair_stats_cells(r0, t)
+    assert np.array_equal(cnt, co) and np.array_equal(h, ho)
+    assert npair == co.sum() // 2
+
+
+def test_full_c2_pair_count_and_momentum(oracle_mod):
+    """Full C2 (998,784 beads): the kernel's interacting-pair count equals the
+    oracle's linked-cell count, and the pair forces sum to ~0."""
+    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    for prec in ("fp64", "mixed"):
+        with Simulation(p, precision=prec) as s:
+            s.set_state(r0, v, t)
+            s.fluid_fluid()
+            npair = s.pair_count()
+            f = s.get_state()["force"]
+        co, _ = _oracle(oracle_mod, p).pair_stats_cells(r0, t)
+        assert npair == co.sum() // 2
+        assert np.abs(f.sum(0)).max() < 1e-6 * np.abs(f).max() * math.sqrt(len(f))
+
+
+def test_profiles_vs_oracle(oracle_mod):
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.profiles_reset(200)
+        s.profiles_accumulate()
+        s.md_steps(3)
+        s.profiles_accumulate()
+        st = s.get_state()
+        nobs, dens, vp, vp2 = s.profiles()
+    o = _oracle(oracle_mod, p)
+    d0 = o.histo(r0, 200) + o.histo(st["r0"], 200)
+    vp0a, vp20a = o.velocity_prof(r0, v, 200)
+    vp0b, vp20b = o.velocity_prof(st["r0"], st["v"], 200)
+    assert nobs == 2 and np.array_equal(dens, d0)
+    assert np.abs(vp - (vp0a + vp0b)).max() < 1e-8
+    assert np.abs(vp2 - (vp20a + vp20b)).max() < 1e-6
+
+
+def test_error_paths():
+    p, t, r0, v = configs.channel_c1()
+    bad = p.replace(boundary=(4.0, 4.0, 20.0))              # < 3 cells in x,y
+    with pytest.raises(Exception, match="too small"):
+        Simulation(bad)
+    with Simulation(p) as s:
+        with pytest.raises(Exception, match="no state"):
+            s.fluid_fluid()
+        t2 = t.copy(); t2[0] = 7
+        with pytest.raises(Exception, match="a_type"):
+            s.set_state(r0, v, t2)
+        s.set_state(r0, v, t)
+        with pytest.raises(Exception, match="inter_type"):
+            s.fluid_wall(1)
