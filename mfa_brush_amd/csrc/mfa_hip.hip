@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -335,7 +336,14 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     if (precision != MFA_PREC_FP64 && precision != MFA_PREC_MIXED) return fail(MFA_E_ARG, "create: bad precision");
     *out = nullptr;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MFA_E_HIP, "no HIP device");
+    hipError_t de = hipGetDeviceCount(&ndev);
+    if (de != hipSuccess || ndev < 1) {
+        const char *hv = getenv("HIP_VISIBLE_DEVICES"), *rv = getenv("ROCR_VISIBLE_DEVICES");
+        return fail(MFA_E_HIP, std::string("no HIP device: hipGetDeviceCount -> ") + hipGetErrorString(de) +
+                                   ", count " + std::to_string(ndev) + ", HIP_VISIBLE_DEVICES=" +
+                                   (hv ? (*hv ? hv : "<empty>") : "<unset>") + ", ROCR_VISIBLE_DEVICES=" +
+                                   (rv ? (*rv ? rv : "<empty>") : "<unset>"));
+    }
     mfa_ctx *c = new mfa_ctx();
     c->n_sys = n_sys;
     c->prec = precision;

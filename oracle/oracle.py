@@ -274,3 +274,64 @@ class Oracle:
             self.lib.orc_cells_free(C.byref(cells))
         en = {k: getattr(e, k) for k, _ in OrcEnergies._fields_}
         return r0, v, a, f, en, fh
+
+
+# ---------------------------------------------------------------------------
+# The reference's own routines (oracle/_ref, built by oracle/ref.mk from the
+# reference sources with the harness oracle/ref_harness.f90).
+# ---------------------------------------------------------------------------
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_available(omp: bool = False) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "libmfa_ref_omp.so" if omp else "libmfa_ref.so"))
+
+
+class RefLib:
+    """One channel in the reference's module state; routines via ref_call."""
+    FLUID_FLUID, FLUID_WALL, INTRA_MOLEC, BENDING, CONSTANT_FORCE, CHECK_FLUID_FLUID = 1, 2, 3, 4, 5, 6
+
+    def __init__(self, p, omp: bool = False, seed: int = 4711):
+        path = os.path.join(REF_DIR, "libmfa_ref_omp.so" if omp else "libmfa_ref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("reference build missing: make -f oracle/ref.mk (%s)" % path)
+        lib = C.CDLL(path)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        lib.ref_setup.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, d, d, d, vp, vp, d, d, d, d, d, d, vp, vp, i]
+        lib.ref_set_state.argtypes = [vp, vp, vp, vp]
+        lib.ref_get_state.argtypes = [vp, vp, vp, vp]
+        lib.ref_call.argtypes = [i, vp, vp]
+        lib.ref_md_steps.argtypes = [i, i, vp]
+        lib.ref_num_threads.restype = i
+        self.lib, self.p = lib, p
+        a = lambda x: np.ascontiguousarray(x, np.float64)
+        self._keep = [a(p.boundary), a(p.epsil), a(p.sigma_2), a(p.range_2), a(p.e_shift), a(p.inv_range_2),
+                      a(p.sigma_wall), a(p.a_wall), a(p.const_force), a(p.mass_type)]
+        k = self._keep
+        lib.ref_setup(p.n_mon, p.n_chain, p.n_mon_d, p.n_chain_d, _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]),
+                      _p(k[4]), _p(k[5]), p.r_cut_max, p.sig, p.friction, _p(k[6]), _p(k[7]), p.a_w,
+                      p.k_chain, p.inv_r_chain_2, p.k_bend, p.alpha_eq, p.dt, _p(k[8]), _p(k[9]), seed)
+
+    def threads(self) -> int:
+        return self.lib.ref_num_threads()
+
+    def set_state(self, r0, v, a, a_type):
+        self._st = [np.ascontiguousarray(r0, np.float64), np.ascontiguousarray(v, np.float64),
+                    np.ascontiguousarray(a, np.float64), np.ascontiguousarray(a_type, np.int32)]
+        self.lib.ref_set_state(*[_p(x) for x in self._st])
+
+    def get_state(self):
+        n = self.p.n_part
+        out = [np.zeros((n, 3)) for _ in range(4)]
+        self.lib.ref_get_state(*[_p(x) for x in out])
+        return dict(zip(("r0", "v", "a", "force"), out))
+
+    def call(self, which):
+        f = np.zeros((self.p.n_part, 3)); e = C.c_double()
+        self.lib.ref_call(which, _p(f), C.byref(e))
+        return f, e.value
+
+    def md_steps(self, n, t0=0):
+        en = np.zeros(6)
+        self.lib.ref_md_steps(n, t0, _p(en))
+        return dict(zip(("v_fluid_fluid", "v_intra_molec", "v_fluid_wall", "v_bend", "v_bend_melt", "t_fluid"), en))
