@@ -60,7 +60,7 @@ struct mfa_ctx {
     Bufs B{};
     int *d_cell_count = nullptr, *d_pre = nullptr, *d_bsum = nullptr;
     int2 *d_crange = nullptr;
-    int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr;
+    int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_perm_tmp = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
     int *d_rebuild_count = nullptr;
     double *d_stage = nullptr;
@@ -142,7 +142,7 @@ static void free_all(mfa_ctx *c) {
     }
     F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.flags); F(c->B.t_idx);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
-    F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_partial);
+    F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_perm_tmp); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -248,8 +248,9 @@ static void enqueue_rebuild(mfa_ctx *c) {
     launch(c->st, k_scan3, gc, c->d_sys, c->B.rebuild, c->d_cell_count, c->d_pre, c->d_bsum, c->d_cell_sys,
            c->d_crange, c->C_tot);
     launch(c->st, k_scatter, g, c->d_sys, c->d_blk_sys, c->B.rebuild, c->d_bin_cell, c->d_bin_rank, c->d_crange,
-           c->d_perm, c->S_tot);
-    launch(c->st, k_cellsort, gc, c->B, c->d_crange, c->d_perm, c->C_tot);
+           c->d_perm_tmp, c->S_tot);
+    launch(c->st, k_rank, g, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
+           (const int *)c->d_perm_tmp, c->d_perm, c->S_tot);
     if (c->prec == MFA_PREC_MIXED)
         launch(c->st, k_permute<true>, g, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     else
@@ -392,6 +393,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_bsum, (c->C_tot + 4 * BLK - 1) / (4 * BLK) + 1);
     ALLOC(c->d_crange, c->C_tot);
     ALLOC(c->d_bin_cell, c->S_tot); ALLOC(c->d_bin_rank, c->S_tot); ALLOC(c->d_perm, c->S_tot);
+    ALLOC(c->d_perm_tmp, c->S_tot);
     ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
     ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);

@@ -87,7 +87,7 @@ struct BendOut { double fp[3], fn[3], v; };
 
 // Harmonic-angle bending centred on c, bending.f90:30-109 (including the
 // small-angle fallback :59-92 and the cos >= 1 guard :93-98).
-__device__ BendOut bend_angle(const SysDev &P, const double4 &rm, const double4 &rc, const double4 &rpn) {
+__device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm, const double4 &rc, const double4 &rpn) {
     BendOut o;
     double rp[3], rn[3];
     rp[0] = mic(rc.x - rm.x, P.L[0], P.invL[0]);
@@ -138,25 +138,26 @@ __device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
 // of nm beads starting at global index g0, in the reference's loop order
 // (centres ascending): +fn of centre m-1, -fp-fn of centre m, +fp of m+1.
 __device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, int par, int g0, int nm, int m,
-                                          const double4 &ri, double3 &F, double &ev) {
-    double4 r[5];
-    for (int q = -2; q <= 2; ++q) {
-        int mm = m + q;
-        r[q + 2] = (q == 0) ? ri : ((mm >= 0 && mm < nm) ? bead_at(B, par, g0 + mm) : ri);
-    }
+                                          const double4 ri, double3 &F, double &ev) {
+    // positions of the beads m-2 .. m+2 of the chain (absent ones are never used)
+    double4 rm2 = ri, rm1 = ri, rp1 = ri, rp2 = ri;
+    if (m >= 2) rm2 = bead_at(B, par, g0 + m - 2);
+    if (m >= 1) rm1 = bead_at(B, par, g0 + m - 1);
+    if (m + 1 < nm) rp1 = bead_at(B, par, g0 + m + 1);
+    if (m + 2 < nm) rp2 = bead_at(B, par, g0 + m + 2);
     if (m - 1 >= 1 && m - 1 <= nm - 2) {
-        BendOut o = bend_angle(P, r[0], r[1], r[2]);
+        BendOut o = bend_angle(P, rm2, rm1, ri);
         F.x += o.fn[0]; F.y += o.fn[1]; F.z += o.fn[2];
     }
     if (m >= 1 && m <= nm - 2) {
-        BendOut o = bend_angle(P, r[1], r[2], r[3]);
+        BendOut o = bend_angle(P, rm1, ri, rp1);
         F.x = F.x - o.fp[0] - o.fn[0];
         F.y = F.y - o.fp[1] - o.fn[1];
         F.z = F.z - o.fp[2] - o.fn[2];
         ev += o.v;
     }
     if (m + 1 >= 1 && m + 1 <= nm - 2) {
-        BendOut o = bend_angle(P, r[2], r[3], r[4]);
+        BendOut o = bend_angle(P, ri, rp1, rp2);
         F.x += o.fp[0]; F.y += o.fp[1]; F.z += o.fp[2];
     }
 }
@@ -375,21 +376,22 @@ __global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys,
     }
 }
 
-// Order each cell by reference bead index (atomics made the rank order
-// arbitrary); insertion sort, cells hold a few tens of beads.
-__global__ __launch_bounds__(BLK) void k_cellsort(Bufs B, const int2 *__restrict__ crange,
-                                                 int *__restrict__ perm, int n_cells) {
+// Order each cell by reference bead index (the atomic ranks of k_bin are
+// arbitrary): every bead counts the beads of its cell with a smaller index
+// and takes that slot.  perm_tmp holds the atomic order, perm the final one.
+__global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                             const int *__restrict__ bin_cell, const int2 *__restrict__ crange,
+                                             const int *__restrict__ perm_tmp, int *__restrict__ perm, int n_slots) {
     if (!*B.rebuild) return;
     MFA_CUR(B);
-    for (int c = blockIdx.x * BLK + threadIdx.x; c < n_cells; c += gridDim.x * BLK) {
-        const int2 rg = crange[c];
-        for (int i = rg.x + 1; i < rg.y; ++i) {
-            const int v = perm[i];
-            const int key = worig(B.pos[par][v].w);
-            int j = i - 1;
-            while (j >= rg.x && worig(B.pos[par][perm[j]].w) > key) { perm[j + 1] = perm[j]; --j; }
-            perm[j + 1] = v;
-        }
+    for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
+        const SysDev &P = sys[blk_sys[k / BLK]];
+        if (k - P.slot_base >= P.n_part) continue;
+        const int2 rg = crange[bin_cell[k]];
+        const int key = worig(B.pos[par][k].w);
+        int rank = 0;
+        for (int m = rg.x; m < rg.y; ++m) rank += worig(B.pos[par][perm_tmp[m]].w) < key ? 1 : 0;
+        perm[rg.x + rank] = k;
     }
 }
 
