@@ -229,10 +229,8 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
     return 0;
 }
 
-template <class K, class... A>
-static void launch(hipStream_t st, K kern, int grid, A... args) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, st, args...);
-}
+// kernels are launched by name (templated kernels parenthesised)
+#define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, st, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // step pieces
@@ -242,26 +240,26 @@ static void enqueue_rebuild(mfa_ctx *c) {
     const int g = std::min(GATED_GRID, std::max(1, c->n_blocks));
     const int gc = std::min(GATED_GRID, (c->C_tot + BLK - 1) / BLK);
     const int nsb = (c->C_tot + 4 * BLK - 1) / (4 * BLK);
-    launch(c->st, k_bin, g, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank, c->S_tot);
-    launch(c->st, k_scan1, nsb, c->B.rebuild, c->d_cell_count, c->C_tot, c->d_pre, c->d_bsum);
-    launch(c->st, k_scan2, 1, c->B.rebuild, c->d_bsum, nsb);
-    launch(c->st, k_scan3, gc, c->d_sys, c->B.rebuild, c->d_cell_count, c->d_pre, c->d_bsum, c->d_cell_sys,
+    LAUNCH(k_bin, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank, c->S_tot);
+    LAUNCH(k_scan1, nsb, c->st, c->B.rebuild, c->d_cell_count, c->C_tot, c->d_pre, c->d_bsum);
+    LAUNCH(k_scan2, 1, c->st, c->B.rebuild, c->d_bsum, nsb);
+    LAUNCH(k_scan3, gc, c->st, c->d_sys, c->B.rebuild, c->d_cell_count, c->d_pre, c->d_bsum, c->d_cell_sys,
            c->d_crange, c->C_tot);
-    launch(c->st, k_scatter, g, c->d_sys, c->d_blk_sys, c->B.rebuild, c->d_bin_cell, c->d_bin_rank, c->d_crange,
+    LAUNCH(k_scatter, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_bin_cell, c->d_bin_rank, c->d_crange,
            c->d_perm_tmp, c->S_tot);
-    launch(c->st, k_rank, g, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
+    LAUNCH(k_rank, g, c->st, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
            (const int *)c->d_perm_tmp, c->d_perm, c->S_tot);
     if (c->prec == MFA_PREC_MIXED)
-        launch(c->st, k_permute<true>, g, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
+        LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     else
-        launch(c->st, k_permute<false>, g, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
+        LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count);
 }
 
 static void enqueue_vv1(mfa_ctx *c) {
     Timer t(c, K_VV1);
-    if (c->prec == MFA_PREC_MIXED) launch(c->st, k_vv1<true>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B);
-    else launch(c->st, k_vv1<false>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B);
+    if (c->prec == MFA_PREC_MIXED) LAUNCH((k_vv1<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
+    else LAUNCH((k_vv1<false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
 }
 
 static void enqueue_pair(mfa_ctx *c) {
@@ -271,19 +269,19 @@ static void enqueue_pair(mfa_ctx *c) {
     const int nb = c->n_blocks;
     if (c->prec == MFA_PREC_MIXED) {
         if (dpd) {
-            if (c->virial) launch(c->st, k_pair32<true, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else launch(c->st, k_pair32<true, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair32<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else LAUNCH((k_pair32<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
         } else {
-            if (c->virial) launch(c->st, k_pair32<false, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else launch(c->st, k_pair32<false, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair32<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else LAUNCH((k_pair32<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
         }
     } else {
         if (dpd) {
-            if (c->virial) launch(c->st, k_pair64<true, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else launch(c->st, k_pair64<true, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair64<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else LAUNCH((k_pair64<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
         } else {
-            if (c->virial) launch(c->st, k_pair64<false, true>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else launch(c->st, k_pair64<false, false>, nb, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair64<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            else LAUNCH((k_pair64<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
         }
     }
 }
@@ -292,9 +290,9 @@ template <int MODE>
 static void enqueue_bonded(mfa_ctx *c) {
     Timer t(c, K_BONDED);
     if (c->prec == MFA_PREC_MIXED)
-        launch(c->st, k_bonded<MODE, true>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        LAUNCH((k_bonded<MODE, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
     else
-        launch(c->st, k_bonded<MODE, false>, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        LAUNCH((k_bonded<MODE, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
 }
 
 static void enqueue_step(mfa_ctx *c) {
@@ -302,6 +300,27 @@ static void enqueue_step(mfa_ctx *c) {
     enqueue_rebuild(c);
     enqueue_pair(c);
     enqueue_bonded<M_FUSED>(c);
+}
+
+// Reads the device guard word (one sync): a nonzero code means a kernel met
+// an out-of-range index and skipped the access.
+static const char *guard_name(int g) {
+    switch (g) {
+        case G_CELL: return "cell id"; case G_RANGE: return "cell slot range"; case G_PERM: return "permutation";
+        case G_ORIG: return "bead index"; case G_WHERE: return "bead slot"; case G_TYPE: return "bead type";
+        case G_HEAD: return "head index"; case G_BIN: return "binned cell"; case G_SLOT: return "neighbour range";
+        default: return "unknown";
+    }
+}
+
+static int check_guard(mfa_ctx *c) {
+    int f[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(f, c->B.flags + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (f[1] != 0)
+        return fail(MFA_E_STATE, std::string("device bounds guard: bad ") + guard_name(f[1]) + " at index " +
+                                     std::to_string(f[0]) + " (state is corrupt; the access was skipped)");
+    return 0;
 }
 
 static int check_ctx(mfa_ctx *c, bool need_state = true) {
@@ -371,6 +390,10 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     if (slots >= (1ll << 28) || cells >= (1ll << 30)) { delete c; return fail(MFA_E_ARG, "too many beads or cells"); }
     c->S_tot = (int)slots; c->N_tot = (int)parts; c->C_tot = (int)cells; c->H_tot = (int)heads;
     c->n_blocks = c->S_tot / BLK;
+    c->B.n_slots = c->S_tot;
+    c->B.n_beads = c->N_tot;
+    c->B.n_cells = c->C_tot;
+    c->B.n_heads = std::max(1, c->H_tot);
     std::vector<int> blk_sys(c->n_blocks), cell_sys(c->C_tot);
     for (int s = 0; s < n_sys; ++s) {
         for (int b = 0; b < c->hs[s].n_blocks; ++b) blk_sys[c->hs[s].block_base + b] = s;
@@ -436,12 +459,13 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
     HIPCHK(hipMemcpy(dv, v, 3 * n * sizeof(double), hipMemcpyHostToDevice));
     if (a) HIPCHK(hipMemcpy(da, a, 3 * n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_istage, a_type, n * sizeof(int), hipMemcpyHostToDevice));
-    launch(c->st, k_upload, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
+    LAUNCH(k_upload, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
            (const double *)da, (const int *)c->d_istage);
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     (void)hipFree(dr); (void)hipFree(dv); if (da) (void)hipFree(da);
+    if (int rc = check_guard(c)) return rc;
     c->uploaded[isys] = 1;
     c->have_state = std::all_of(c->uploaded.begin(), c->uploaded.end(), [](char u) { return u != 0; });
     // cell list of the new state (init_config.f90:261-272); pending until every
@@ -461,15 +485,16 @@ int mfa_hip_get_state(mfa_ctx *c, int32_t isys, double *r0, double *v, double *a
     if (isys < 0 || isys >= c->n_sys) return fail(MFA_E_ARG, "get_state: bad channel");
     const SysDev &d = c->hs[isys];
     const size_t n = d.n_part;
+    if (int rc = check_guard(c)) return rc;
     double *outs[4] = {r0, v, a, force};
     for (int w = 0; w < 4; ++w) {
         if (!outs[w]) continue;
-        launch(c->st, k_download, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, c->d_stage, w);
+        LAUNCH(k_download, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, c->d_stage, w);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(outs[w], c->d_stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
     }
-    return 0;
+    return check_guard(c);
 }
 
 int mfa_hip_set_time(mfa_ctx *c, int64_t t_idx) {
@@ -602,7 +627,7 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
 int mfa_hip_synchronize(mfa_ctx *c) {
     if (int rc = check_ctx(c, false)) return rc;
     HIPCHK(hipStreamSynchronize(c->st));
-    return 0;
+    return check_guard(c);
 }
 
 int mfa_hip_get_energies(mfa_ctx *c, int32_t isys, mfa_energies *out) {
@@ -614,6 +639,7 @@ int mfa_hip_get_energies(mfa_ctx *c, int32_t isys, mfa_energies *out) {
     double e[NPART];
     HIPCHK(hipMemcpyAsync(e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = check_guard(c)) return rc;
     const mfa_system_params &p = c->hp[isys];
     out->v_fluid_fluid = 0.5 * e[C_VFF];
     out->v_fluid_wall = e[C_VWALL];
@@ -678,7 +704,7 @@ static int brute(mfa_ctx *c, int isys, double *e_sum, int32_t *count, uint64_t *
         double r_time = (double)t * p.dt;
         if (r_time < p.r_2_min_time) r_2_min = (1.0 - r_time / p.r_2_min_time) * p.r_2_min_init;
     }
-    launch(c->st, k_brute, (int)((n + BLK - 1) / BLK), c->d_sys, isys, c->B, r_2_min, de, dc, dh);
+    LAUNCH(k_brute, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, r_2_min, de, dc, dh);
     HIPCHK(hipGetLastError());
     std::vector<double> he(n);
     HIPCHK(hipMemcpyAsync(he.data(), de, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -689,7 +715,7 @@ static int brute(mfa_ctx *c, int isys, double *e_sum, int32_t *count, uint64_t *
     for (size_t i = 0; i < n; ++i) s += he[i];
     if (e_sum) *e_sum = s;
     (void)hipFree(de); (void)hipFree(dc); (void)hipFree(dh);
-    return 0;
+    return check_guard(c);
 }
 
 int mfa_hip_check_fluid_fluid(mfa_ctx *c, int32_t isys, double *v) {
@@ -753,9 +779,9 @@ int mfa_hip_profiles_reset(mfa_ctx *c, int32_t n_bins) {
 int mfa_hip_profiles_accumulate(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     if (c->prof_nb < 1) return fail(MFA_E_STATE, "profiles_accumulate: call profiles_reset first");
-    launch(c->st, k_prof_bin, c->n_blocks, c->d_sys, c->d_blk_sys, c->B, c->prof_nb, c->d_pcnt, c->d_pvs, c->d_pv2s);
+    LAUNCH(k_prof_bin, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->prof_nb, c->d_pcnt, c->d_pvs, c->d_pv2s);
     const int nc = c->n_sys * 2 * c->prof_nb, nv = nc * 3;
-    launch(c->st, k_prof_fold, (nv + BLK - 1) / BLK, nc, nv, c->d_pcnt, c->d_pvs, c->d_pv2s, c->d_pdens, c->d_pvp,
+    LAUNCH(k_prof_fold, (nv + BLK - 1) / BLK, c->st, nc, nv, c->d_pcnt, c->d_pvs, c->d_pv2s, c->d_pdens, c->d_pvp,
            c->d_pvp2);
     HIPCHK(hipGetLastError());
     c->prof_nobs += 1;

@@ -55,11 +55,28 @@ struct Bufs {
     int *where;
     int *parity;
     int *rebuild;
-    int *flags;
+    int *flags;         // [0..3] anomaly counters, [6] offending index, [7] first guard code
     long long *t_idx;
+    int n_slots, n_beads, n_cells, n_heads;   // buffer extents, for the guards
 };
 
 #define MFA_CUR(B) const int par = *(B).parity; (void)par
+
+// Device-side guards.  Every data-dependent index (cell ids, cell ranges,
+// permutations, bead identities, chain neighbours) is checked against the
+// extent of the buffer it addresses before it is used; a bad index records
+// the first failure (code, index) in flags[7], flags[6] and the access is
+// skipped, so corrupted data is reported by the host instead of faulting
+// the device.
+enum GuardCode { G_CELL = 1, G_RANGE, G_PERM, G_ORIG, G_WHERE, G_TYPE, G_HEAD, G_BIN, G_SLOT };
+
+__device__ __forceinline__ bool guard(const Bufs &B, bool ok, int code, long long idx) {
+    if (!ok) {
+        if (atomicCAS(&B.flags[7], 0, code) == 0) B.flags[6] = (int)idx;
+    }
+    return ok;
+}
+__device__ __forceinline__ bool in_range(long long v, long long n) { return v >= 0 && v < n; }
 
 __device__ __forceinline__ uint64_t wbits(double w) { return (uint64_t)__double_as_longlong(w); }
 __device__ __forceinline__ int wtype(double w) { return (int)(wbits(w) & 7ull); }

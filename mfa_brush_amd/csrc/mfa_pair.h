@@ -26,7 +26,7 @@ __device__ __forceinline__ void pair64(const SysDev &P, const double *tab, doubl
                                        double4 pj, const double4 *__restrict__ vel, int m,
                                        double r_2_min, uint64_t t_idx, PairAcc &a) {
     const int tj = wtype(pj.w);
-    const int tt = (ti - 1) + 4 * (tj - 1);
+    const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
     double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
     double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
     double dz = zi - pj.z;
@@ -146,9 +146,11 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     if (active) {
         const double4 pi = pos[k];
         const int ti = wtype(pi.w), oi = worig(pi.w);
+        guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         double vxi = 0.0, vyi = 0.0, vzi = 0.0;
         if (DPD) { double4 v = vel[k]; vxi = v.x; vyi = v.y; vzi = v.z; }
-        const int c = cell_of[k] - P.cell_base;
+        int c = cell_of[k] - P.cell_base;
+        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
         const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         const int S = P.S;
@@ -169,6 +171,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                     const int c0 = seg == 0 ? seg_a0 : seg_b0, c1 = seg == 0 ? seg_a1 : seg_b1;
                     if (c1 < c0) continue;
                     const int m0 = crange[row + c0].x, m1 = crange[row + c1].y;
+                    if (!guard(B, m0 >= 0 && m0 <= m1 && m1 <= B.n_slots, G_SLOT, k)) continue;
                     for (int m = m0; m < m1; ++m) {
                         if (m == k) continue;
                         pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m,
@@ -248,9 +251,11 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
     if (active) {
         const float4 pf = posf[k];
         const int ti = fwtype(pf.w), oi = fworig(pf.w);
+        guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
         if (DPD) vi = velf[k];
-        const int c = cell_of[k] - P.cell_base;
+        int c = cell_of[k] - P.cell_base;
+        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
         const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         const int S = P.S;
@@ -270,7 +275,8 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
                     x2 += (x2 < 0) ? nx : 0;
                     x2 -= (x2 >= nx) ? nx : 0;
                     const float qx = pf.x - (float)dx * lx;
-                    const int2 rg = crange[row + x2];
+                    int2 rg = crange[row + x2];
+                    if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
                     float cfx = 0.f, cfy = 0.f, cfz = 0.f, ce = 0.f;
                     float cw[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     for (int m = rg.x; m < rg.y; ++m) {
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
                         const float ddx = qx - pj.x, ddy = qy - pj.y, ddz = qz - pj.z;
                         float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         const int tj = fwtype(pj.w);
-                        const int tt = (ti - 1) + 4 * (tj - 1);
+                        const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
                         const bool maybe_dpd = DPD && P.dpd_cut_off && r2 < rdpd2 * 1.0001f;
                         if (!(r2 < tf[48 + tt]) && !maybe_dpd) continue;
                         bool lj = r2 < tf[32 + tt];

@@ -49,6 +49,7 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
     }
     if (MIXED) {
         const int c = B.cell_of[par][k] - P.cell_base;
+        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) return;
         const int nx = P.nc[0], ny = P.nc[1];
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
@@ -131,7 +132,10 @@ __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm
 enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64 };
 
 __device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
-    return B.pos[par][B.where[gorig]];
+    if (!guard(B, in_range(gorig, B.n_beads), G_ORIG, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
+    const int k = B.where[gorig];
+    if (!guard(B, in_range(k, B.n_slots), G_WHERE, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
+    return B.pos[par][k];
 }
 
 // Bending contributions to the bead at chain position m (0-based) of a chain
@@ -182,6 +186,8 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         const int ti = wtype(ri.w);
         const int g = worig(ri.w);
         const int o = g - P.part_base;
+        guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
+        guard(B, in_range(o, P.n_part), G_ORIG, k);
         double4 f4 = B.frc[k];
         double3 F = make_double3(f4.x, f4.y, f4.z);
         const bool brush = o < P.pd;
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         const int g0 = g - m;
         if (MODE & (M_WALL | M_FUSED)) {
             // fluid_wall.f90:93-121; energy with the global a_w (Q9)
-            const double sw = P.sigma_wall[ti - 1], aw = P.a_wall[ti - 1];
+            const double sw = P.sigma_wall[(ti - 1) & 3], aw = P.a_wall[(ti - 1) & 3];
             double inv_z = 1.0 / ri.z;
             double rd = sw * inv_z;
             e_wall += fabs(aw) * ipow(rd, 9) - P.a_w * ipow(rd, 3);
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         }
         if ((MODE & M_CONST) || (FUSED && P.poiseuille)) {
             if (ti != 1) {
-                const double ms = P.mass_type[ti - 1];
+                const double ms = P.mass_type[(ti - 1) & 3];
                 F.x += ms * P.const_force[0];
                 F.y += ms * P.const_force[1];
                 F.z += ms * P.const_force[2];
@@ -232,13 +238,13 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         }
         if (MODE & (M_VV2 | M_FUSED)) {
             const bool head = brush && m == 0;
-            if (head) {
+            if (head && guard(B, in_range(P.head_base + ol / P.n_mon, B.n_heads), G_HEAD, k)) {
                 double *fh = f_heads + 3 * (size_t)(P.head_base + ol / P.n_mon);
                 fh[0] = F.x; fh[1] = F.y; fh[2] = F.z;
             }
             double4 v = B.vel[par][k];
             double4 a;
-            const double im = P.inv_mass_type[ti - 1];
+            const double im = P.inv_mass_type[(ti - 1) & 3];
             {
 #pragma clang fp contract(off)
                 a.x = F.x * im; a.y = F.y * im; a.z = F.z * im; a.w = 0.0;
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
                 a.x = a.y = a.z = 0.0;
                 F.x = F.y = F.z = 0.0;
             }
-            ke = P.mass_type[ti - 1] * (v.x * v.x + v.y * v.y + v.z * v.z);
+            ke = P.mass_type[(ti - 1) & 3] * (v.x * v.x + v.y * v.y + v.z * v.z);
             B.vel[par][k] = v;
             B.acc[par][k] = a;
             if (MIXED) B.velf[par][k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
@@ -305,6 +311,7 @@ __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, con
         if (k - P.slot_base >= P.n_part) continue;
         const double4 r = B.pos[par][k];
         const int c = cell_of_pos(P, r.x, r.y, r.z);
+        if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         bin_cell[k] = c;
         bin_rank[k] = atomicAdd(&cell_count[c], 1);
     }
@@ -365,14 +372,18 @@ __global__ __launch_bounds__(BLK) void k_scan3(const SysDev *__restrict__ sys, c
 }
 
 __global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
-                                                const int *__restrict__ rebuild, const int *__restrict__ bin_cell,
+                                                Bufs B, const int *__restrict__ bin_cell,
                                                 const int *__restrict__ bin_rank, const int2 *__restrict__ crange,
                                                 int *__restrict__ perm, int n_slots) {
-    if (!*rebuild) return;
+    if (!*B.rebuild) return;
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const SysDev &P = sys[blk_sys[k / BLK]];
         if (k - P.slot_base >= P.n_part) continue;
-        perm[crange[bin_cell[k]].x + bin_rank[k]] = k;
+        const int c = bin_cell[k];
+        if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
+        const int m = crange[c].x + bin_rank[k];
+        if (!guard(B, in_range(m, n_slots), G_RANGE, k)) continue;
+        perm[m] = k;
     }
 }
 
@@ -387,11 +398,19 @@ __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, co
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const SysDev &P = sys[blk_sys[k / BLK]];
         if (k - P.slot_base >= P.n_part) continue;
-        const int2 rg = crange[bin_cell[k]];
+        const int c = bin_cell[k];
+        if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
+        const int2 rg = crange[c];
+        if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= n_slots, G_RANGE, k)) continue;
         const int key = worig(B.pos[par][k].w);
         int rank = 0;
-        for (int m = rg.x; m < rg.y; ++m) rank += worig(B.pos[par][perm_tmp[m]].w) < key ? 1 : 0;
-        perm[rg.x + rank] = k;
+        bool ok = true;
+        for (int m = rg.x; m < rg.y; ++m) {
+            const int q = perm_tmp[m];
+            if (!guard(B, in_range(q, n_slots), G_PERM, m)) { ok = false; break; }
+            rank += worig(B.pos[par][q].w) < key ? 1 : 0;
+        }
+        if (ok && guard(B, rank < rg.y - rg.x, G_RANGE, k)) perm[rg.x + rank] = k;
     }
 }
 
@@ -406,7 +425,9 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         const SysDev &P = sys[blk_sys[m / BLK]];
         if (m - P.slot_base >= P.n_part) continue;
         const int old = perm[m];
+        if (!guard(B, in_range(old, n_slots), G_PERM, m)) continue;
         const double4 r = B.pos[par][old];
+        if (!guard(B, in_range(worig(r.w), B.n_beads), G_ORIG, m)) continue;
         const double4 v = B.vel[par][old];
         B.pos[nw][m] = r;
         B.vel[nw][m] = v;
@@ -417,6 +438,7 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         B.where[worig(r.w)] = m;
         if (MIXED) {
             const int c = cg - P.cell_base;
+            if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, m)) continue;
             const int nx = P.nc[0], ny = P.nc[1];
             const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
             double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
@@ -461,6 +483,7 @@ __global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= P.n_part) return;
     const int k = B.where[P.part_base + i];
+    if (!guard(B, in_range(k, B.n_slots), G_WHERE, i)) return;
     double4 q = which == 0 ? B.pos[par][k] : which == 1 ? B.vel[par][k] : which == 2 ? B.acc[par][k] : B.frc[k];
     out[3 * i] = q.x; out[3 * i + 1] = q.y; out[3 * i + 2] = q.z;
 }
@@ -484,14 +507,14 @@ __global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, i
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
     const bool act = i < P.n_part;
-    double4 ri = act ? B.pos[par][B.where[P.part_base + i]] : make_double4(0, 0, 0, 0);
+    double4 ri = act ? bead_at(B, par, P.part_base + i) : make_double4(0, 0, 0, 0);
     const int ti = act ? wtype(ri.w) : 1;
     double e = 0.0;
     int cnt = 0;
     unsigned long long h = 0ull;
     for (int j0 = 0; j0 < P.n_part; j0 += BLK) {
         const int jj = j0 + threadIdx.x;
-        tile[threadIdx.x] = jj < P.n_part ? B.pos[par][B.where[P.part_base + jj]] : make_double4(0, 0, 0, 0);
+        tile[threadIdx.x] = jj < P.n_part ? bead_at(B, par, P.part_base + jj) : make_double4(0, 0, 0, 0);
         __syncthreads();
         const int jn = min(BLK, P.n_part - j0);
         if (act) {
@@ -500,7 +523,7 @@ __global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, i
                 if (j == i) continue;
                 const double4 rj = tile[q];
                 const int tj = wtype(rj.w);
-                const int tt = (ti - 1) + 4 * (tj - 1);
+                const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
                 double dx = mic(ri.x - rj.x, P.L[0], P.invL[0]);
                 double dy = mic(ri.y - rj.y, P.L[1], P.invL[1]);
                 double dz = ri.z - rj.z;
