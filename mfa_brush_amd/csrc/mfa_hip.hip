@@ -65,6 +65,7 @@ struct mfa_ctx {
     int *d_rebuild_count = nullptr;
     double *d_stage = nullptr;
     int *d_istage = nullptr;
+    char *h_stage = nullptr;      // pinned staging buffer for every host<->device copy
     // profiles
     int prof_nb = 0, prof_nobs = 0;
     unsigned int *d_pcnt = nullptr;
@@ -134,6 +135,43 @@ static int dalloc(T **p, size_t n) {
     return 0;
 }
 
+// Host<->device transfers.  The stream is synchronised first, so an error of
+// an earlier kernel is reported by that synchronisation; the data then moves
+// in chunks between device memory and the context's pinned staging buffer
+// (DMA only ever touches pinned host memory), and the CPU copies between the
+// staging buffer and the caller's memory.
+constexpr size_t STAGE_BYTES = 16u << 20;
+
+static int sync_for(mfa_ctx *c, const char *what) {
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(MFA_E_HIP, std::string("device work before ") + what + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+static int to_host(mfa_ctx *c, void *dst, const void *src, size_t bytes, const char *what) {
+    if (int rc = sync_for(c, what)) return rc;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        hipError_t e = hipMemcpyAsync(c->h_stage, (const char *)src + off, n, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) return fail(MFA_E_HIP, std::string("device-to-host copy of ") + what + ": " + hipGetErrorString(e));
+        memcpy((char *)dst + off, c->h_stage, n);
+    }
+    return 0;
+}
+
+static int to_device(mfa_ctx *c, void *dst, const void *src, size_t bytes, const char *what) {
+    if (int rc = sync_for(c, what)) return rc;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        memcpy(c->h_stage, (const char *)src + off, n);
+        hipError_t e = hipMemcpyAsync((char *)dst + off, c->h_stage, n, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) return fail(MFA_E_HIP, std::string("host-to-device copy of ") + what + ": " + hipGetErrorString(e));
+    }
+    return 0;
+}
+
 static void free_all(mfa_ctx *c) {
     auto F = [](void *p) { if (p) (void)hipFree(p); };
     for (int h = 0; h < 2; ++h) {
@@ -145,6 +183,7 @@ static void free_all(mfa_ctx *c) {
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_perm_tmp); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -315,8 +354,7 @@ static const char *guard_name(int g) {
 
 static int check_guard(mfa_ctx *c) {
     int f[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(f, c->B.flags + 6, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, f, c->B.flags + 6, 2 * sizeof(int), "guard flags")) return rc;
     if (f[1] != 0)
         return fail(MFA_E_STATE, std::string("device bounds guard: bad ") + guard_name(f[1]) + " at index " +
                                      std::to_string(f[0]) + " (state is corrupt; the access was skipped)");
@@ -404,6 +442,9 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c; return fail(MFA_E_HIP, "stream creation failed");
     }
+    if (hipHostMalloc((void **)&c->h_stage, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+        c->h_stage = nullptr; free_all(c); delete c; return fail(MFA_E_HIP, "pinned staging allocation failed");
+    }
     for (int h = 0; h < 2; ++h) {
         ALLOC(c->B.pos[h], c->S_tot); ALLOC(c->B.vel[h], c->S_tot); ALLOC(c->B.acc[h], c->S_tot);
         ALLOC(c->B.posb[h], c->S_tot); ALLOC(c->B.cell_of[h], c->S_tot);
@@ -421,14 +462,15 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
 #undef ALLOC
-    if (hipMemcpy(c->d_sys, c->hs.data(), n_sys * sizeof(SysDev), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_blk_sys, blk_sys.data(), c->n_blocks * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_cell_sys, cell_sys.data(), c->C_tot * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-        free_all(c); delete c; return fail(MFA_E_HIP, "parameter upload failed");
-    }
+    // the zero fills of dalloc ran on the null stream: finish them before
+    // anything is enqueued on the context's non-blocking stream
+    if (hipDeviceSynchronize() != hipSuccess) { free_all(c); delete c; return fail(MFA_E_HIP, "allocation sync failed"); }
     long long t0 = sys[0].t_idx0;
-    if (hipMemcpy(c->B.t_idx, &t0, sizeof(long long), hipMemcpyHostToDevice) != hipSuccess) {
-        free_all(c); delete c; return fail(MFA_E_HIP, "time upload failed");
+    if ((rc = to_device(c, c->d_sys, c->hs.data(), n_sys * sizeof(SysDev), "channel parameters")) ||
+        (rc = to_device(c, c->d_blk_sys, blk_sys.data(), c->n_blocks * sizeof(int), "block table")) ||
+        (rc = to_device(c, c->d_cell_sys, cell_sys.data(), c->C_tot * sizeof(int), "cell table")) ||
+        (rc = to_device(c, c->B.t_idx, &t0, sizeof(long long), "time index"))) {
+        free_all(c); delete c; return rc;
     }
     *out = c;
     return 0;
@@ -450,22 +492,24 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
     const size_t n = d.n_part;
     for (size_t i = 0; i < n; ++i)
         if (a_type[i] < 1 || a_type[i] > 4) return fail(MFA_E_ARG, "set_state: a_type outside 1..4");
+    // r0, v, a, a_type go through d_stage (3 n doubles) one array at a time
     double *dr = nullptr, *dv = nullptr, *da = nullptr;
-    HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipMalloc((void **)&dr, 3 * n * sizeof(double)));
     HIPCHK(hipMalloc((void **)&dv, 3 * n * sizeof(double)));
     if (a) HIPCHK(hipMalloc((void **)&da, 3 * n * sizeof(double)));
-    HIPCHK(hipMemcpy(dr, r0, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dv, v, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-    if (a) HIPCHK(hipMemcpy(da, a, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_istage, a_type, n * sizeof(int), hipMemcpyHostToDevice));
-    LAUNCH(k_upload, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
-           (const double *)da, (const int *)c->d_istage);
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->st));
+    int rc = to_device(c, dr, r0, 3 * n * sizeof(double), "r0");
+    if (!rc) rc = to_device(c, dv, v, 3 * n * sizeof(double), "v");
+    if (!rc && a) rc = to_device(c, da, a, 3 * n * sizeof(double), "a");
+    if (!rc) rc = to_device(c, c->d_istage, a_type, n * sizeof(int), "a_type");
+    if (!rc) {
+        LAUNCH(k_upload, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
+               (const double *)da, (const int *)c->d_istage);
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 1);
+        rc = sync_for(c, "state upload");
+    }
     (void)hipFree(dr); (void)hipFree(dv); if (da) (void)hipFree(da);
-    if (int rc = check_guard(c)) return rc;
+    if (rc) return rc;
+    if (int rg = check_guard(c)) return rg;
     c->uploaded[isys] = 1;
     c->have_state = std::all_of(c->uploaded.begin(), c->uploaded.end(), [](char u) { return u != 0; });
     // cell list of the new state (init_config.f90:261-272); pending until every
@@ -491,8 +535,8 @@ int mfa_hip_get_state(mfa_ctx *c, int32_t isys, double *r0, double *v, double *a
         if (!outs[w]) continue;
         LAUNCH(k_download, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, c->d_stage, w);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(outs[w], c->d_stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
+        static const char *names[4] = {"r0", "v", "a", "force"};
+        if (int rc = to_host(c, outs[w], c->d_stage, 3 * n * sizeof(double), names[w])) return rc;
     }
     return check_guard(c);
 }
@@ -500,17 +544,14 @@ int mfa_hip_get_state(mfa_ctx *c, int32_t isys, double *r0, double *v, double *a
 int mfa_hip_set_time(mfa_ctx *c, int64_t t_idx) {
     if (int rc = check_ctx(c, false)) return rc;
     long long t = t_idx;
-    HIPCHK(hipMemcpyAsync(c->B.t_idx, &t, sizeof(long long), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    return 0;
+    return to_device(c, c->B.t_idx, &t, sizeof(long long), "time index");
 }
 
 int mfa_hip_get_time(mfa_ctx *c, int64_t *t_idx) {
     if (int rc = check_ctx(c, false)) return rc;
     if (!t_idx) return fail(MFA_E_ARG, "get_time: null");
     long long t = 0;
-    HIPCHK(hipMemcpyAsync(&t, c->B.t_idx, sizeof(long long), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, &t, c->B.t_idx, sizeof(long long), "time index")) return rc;
     *t_idx = t;
     return 0;
 }
@@ -531,8 +572,7 @@ int mfa_hip_check_skin(mfa_ctx *c, int32_t *f_skin) {
     // verlet_positions already rebuilt the cells when the skin test fired;
     // report how many rebuilds happened so far.
     int v = 0;
-    HIPCHK(hipMemcpyAsync(&v, c->d_rebuild_count, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, &v, c->d_rebuild_count, sizeof(int), "rebuild count")) return rc;
     *f_skin = v;
     return 0;
 }
@@ -637,8 +677,7 @@ int mfa_hip_get_energies(mfa_ctx *c, int32_t isys, mfa_energies *out) {
                        c->d_energies);
     HIPCHK(hipGetLastError());
     double e[NPART];
-    HIPCHK(hipMemcpyAsync(e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), "energies")) return rc;
     if (int rc = check_guard(c)) return rc;
     const mfa_system_params &p = c->hp[isys];
     out->v_fluid_fluid = 0.5 * e[C_VFF];
@@ -657,8 +696,7 @@ int mfa_hip_get_pair_count(mfa_ctx *c, int32_t isys, int64_t *n_pairs) {
     hipLaunchKernelGGL(k_reduce, dim3(c->n_sys), dim3(BLK), 0, c->st, c->d_sys, (const double *)c->d_partial,
                        c->d_energies);
     double e[NPART];
-    HIPCHK(hipMemcpyAsync(e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, e, c->d_energies + (size_t)isys * NPART, NPART * sizeof(double), "pair count")) return rc;
     *n_pairs = (int64_t)llround(0.5 * e[C_NPAIR]);
     return 0;
 }
@@ -668,19 +706,15 @@ int mfa_hip_get_f_on_heads(mfa_ctx *c, int32_t isys, double *f) {
     if (isys < 0 || isys >= c->n_sys || !f) return fail(MFA_E_ARG, "get_f_on_heads: bad argument");
     const SysDev &d = c->hs[isys];
     if (d.n_chain == 0) return 0;
-    HIPCHK(hipMemcpyAsync(f, c->d_fheads + 3 * (size_t)d.head_base, 3 * (size_t)d.n_chain * sizeof(double),
-                          hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    return 0;
+    return to_host(c, f, c->d_fheads + 3 * (size_t)d.head_base, 3 * (size_t)d.n_chain * sizeof(double), "f_on_heads");
 }
 
 int mfa_hip_get_flags(mfa_ctx *c, mfa_flags *out) {
     if (int rc = check_ctx(c, false)) return rc;
     if (!out) return fail(MFA_E_ARG, "get_flags: null");
     int f[8], rb = 0;
-    HIPCHK(hipMemcpyAsync(f, c->B.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(&rb, c->d_rebuild_count, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, f, c->B.flags, 8 * sizeof(int), "flags")) return rc;
+    if (int rc = to_host(c, &rb, c->d_rebuild_count, sizeof(int), "rebuild count")) return rc;
     out->nonfinite = f[0];
     out->out_of_box_z = f[1];
     out->fene_stretched = f[2];
@@ -696,8 +730,7 @@ static int brute(mfa_ctx *c, int isys, double *e_sum, int32_t *count, uint64_t *
     HIPCHK(hipMalloc((void **)&dc, n * sizeof(int)));
     HIPCHK(hipMalloc((void **)&dh, n * sizeof(unsigned long long)));
     long long t = 0;
-    HIPCHK(hipMemcpyAsync(&t, c->B.t_idx, sizeof(long long), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, &t, c->B.t_idx, sizeof(long long), "time index")) return rc;
     const mfa_system_params &p = c->hp[isys];
     double r_2_min = 0.0;
     if (p.force_switch_on) {
@@ -707,10 +740,10 @@ static int brute(mfa_ctx *c, int isys, double *e_sum, int32_t *count, uint64_t *
     LAUNCH(k_brute, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, r_2_min, de, dc, dh);
     HIPCHK(hipGetLastError());
     std::vector<double> he(n);
-    HIPCHK(hipMemcpyAsync(he.data(), de, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    if (count) HIPCHK(hipMemcpyAsync(count, dc, n * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    if (hash) HIPCHK(hipMemcpyAsync(hash, dh, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    int rc = to_host(c, he.data(), de, n * sizeof(double), "brute-force energies");
+    if (!rc && count) rc = to_host(c, count, dc, n * sizeof(int), "pair counts");
+    if (!rc && hash) rc = to_host(c, hash, dh, n * sizeof(unsigned long long), "pair hashes");
+    if (rc) { (void)hipFree(de); (void)hipFree(dc); (void)hipFree(dh); return rc; }
     double s = 0.0;
     for (size_t i = 0; i < n; ++i) s += he[i];
     if (e_sum) *e_sum = s;
@@ -738,12 +771,11 @@ int mfa_hip_get_cells(mfa_ctx *c, int32_t isys, int32_t *n_cells, int32_t *cell_
     const SysDev &d = c->hs[isys];
     if (n_cells) for (int k = 0; k < 3; ++k) n_cells[k] = d.nc[k];
     std::vector<int2> rg(d.n_cells_tot);
-    HIPCHK(hipMemcpyAsync(rg.data(), c->d_crange + d.cell_base, d.n_cells_tot * sizeof(int2), hipMemcpyDeviceToHost, c->st));
+    if (int rc = to_host(c, rg.data(), c->d_crange + d.cell_base, d.n_cells_tot * sizeof(int2), "cell ranges")) return rc;
     int par = 0;
-    HIPCHK(hipMemcpyAsync(&par, c->B.parity, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = to_host(c, &par, c->B.parity, sizeof(int), "parity")) return rc;
     std::vector<double4> pos(d.n_part);
-    HIPCHK(hipMemcpy(pos.data(), c->B.pos[par] + d.slot_base, d.n_part * sizeof(double4), hipMemcpyDeviceToHost));
+    if (int rc = to_host(c, pos.data(), c->B.pos[par] + d.slot_base, d.n_part * sizeof(double4), "positions")) return rc;
     for (int q = 0; q < d.n_cells_tot; ++q) {
         if (cell_start) cell_start[q] = rg[q].x - d.slot_base;
         if (cell_count) cell_count[q] = rg[q].y - rg[q].x;
@@ -771,6 +803,7 @@ int mfa_hip_profiles_reset(mfa_ctx *c, int32_t n_bins) {
     if (int rc = dalloc(&c->d_pdens, nc)) return rc;
     if (int rc = dalloc(&c->d_pvp, nv)) return rc;
     if (int rc = dalloc(&c->d_pvp2, nv)) return rc;
+    HIPCHK(hipDeviceSynchronize());
     c->prof_nb = n_bins;
     c->prof_nobs = 0;
     return 0;
@@ -796,10 +829,10 @@ int mfa_hip_profiles_get(mfa_ctx *c, int32_t isys, int32_t *n_obs, double *dens_
     // device layout: dens[s][sp][bin], v[s][sp][d][bin]; reference layout:
     // dens_z(bin, sp), v_prof(bin, d, sp)
     std::vector<double> hd(2 * nb), hv(6 * nb), hv2(6 * nb);
-    HIPCHK(hipMemcpyAsync(hd.data(), c->d_pdens + (size_t)isys * 2 * nb, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(hv.data(), c->d_pvp + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(hv2.data(), c->d_pvp2 + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    int rc = to_host(c, hd.data(), c->d_pdens + (size_t)isys * 2 * nb, 2 * nb * sizeof(double), "density profile");
+    if (!rc) rc = to_host(c, hv.data(), c->d_pvp + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), "velocity profile");
+    if (!rc) rc = to_host(c, hv2.data(), c->d_pvp2 + (size_t)isys * 6 * nb, 6 * nb * sizeof(double), "velocity^2 profile");
+    if (rc) return rc;
     if (dens_z) memcpy(dens_z, hd.data(), 2 * nb * sizeof(double));
     for (int sp = 0; sp < 2; ++sp)
         for (int d = 0; d < 3; ++d)
