@@ -533,7 +533,11 @@ int mfa_hip_get_state(mfa_ctx *c, int32_t isys, double *r0, double *v, double *a
     double *outs[4] = {r0, v, a, force};
     for (int w = 0; w < 4; ++w) {
         if (!outs[w]) continue;
-        LAUNCH(k_download, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, c->d_stage, w);
+        const int gd = (int)((n + BLK - 1) / BLK);
+        if (w == 0) LAUNCH((k_download<0>), gd, c->st, c->d_sys, isys, c->B, c->d_stage);
+        else if (w == 1) LAUNCH((k_download<1>), gd, c->st, c->d_sys, isys, c->B, c->d_stage);
+        else if (w == 2) LAUNCH((k_download<2>), gd, c->st, c->d_sys, isys, c->B, c->d_stage);
+        else LAUNCH((k_download<3>), gd, c->st, c->d_sys, isys, c->B, c->d_stage);
         HIPCHK(hipGetLastError());
         static const char *names[4] = {"r0", "v", "a", "force"};
         if (int rc = to_host(c, outs[w], c->d_stage, 3 * n * sizeof(double), names[w])) return rc;

@@ -61,6 +61,12 @@ struct Bufs {
 };
 
 #define MFA_CUR(B) const int par = *(B).parity; (void)par
+// The two halves are selected with constant indices (no dynamic indexing
+// into the by-value kernel-argument struct): the AMDGPU backend of this
+// toolchain left the pointer register undefined on one branch of a runtime
+// switch over such dynamically indexed kernel-argument pointers.
+#define CUR(arr) (par ? (arr)[1] : (arr)[0])
+#define NXT(arr) (par ? (arr)[0] : (arr)[1])
 
 // Device-side guards.  Every data-dependent index (cell ids, cell ranges,
 // permutations, bead identities, chain neighbours) is checked against the

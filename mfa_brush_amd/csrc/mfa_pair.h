@@ -118,9 +118,9 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                                                Bufs B, const int2 *__restrict__ crange,
                                                double *__restrict__ partial, int assign) {
     MFA_CUR(B);
-    const double4 *__restrict__ pos = B.pos[par];
-    const double4 *__restrict__ vel = B.vel[par];
-    const int *__restrict__ cell_of = B.cell_of[par];
+    const double4 *__restrict__ pos = CUR(B.pos);
+    const double4 *__restrict__ vel = CUR(B.vel);
+    const int *__restrict__ cell_of = CUR(B.cell_of);
     double4 *__restrict__ frc = B.frc;
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;
@@ -208,10 +208,10 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
                                                Bufs B, const int2 *__restrict__ crange,
                                                double *__restrict__ partial, int assign) {
     MFA_CUR(B);
-    const double4 *__restrict__ pos = B.pos[par];
-    const float4 *__restrict__ posf = B.posf[par];
-    const float4 *__restrict__ velf = B.velf[par];
-    const int *__restrict__ cell_of = B.cell_of[par];
+    const double4 *__restrict__ pos = CUR(B.pos);
+    const float4 *__restrict__ posf = CUR(B.posf);
+    const float4 *__restrict__ velf = CUR(B.velf);
+    const int *__restrict__ cell_of = CUR(B.cell_of);
     double4 *__restrict__ frc = B.frc;
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;

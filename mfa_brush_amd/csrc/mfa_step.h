@@ -20,9 +20,9 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
     const int k = blockIdx.x * BLK + threadIdx.x;
     if (threadIdx.x == 0 && P.skin_mode == 0) *B.rebuild = 1;
     if (k - P.slot_base >= P.n_part) return;
-    double4 r = B.pos[par][k];
-    double4 v = B.vel[par][k];
-    const double4 a = B.acc[par][k];
+    double4 r = CUR(B.pos)[k];
+    double4 v = CUR(B.vel)[k];
+    const double4 a = CUR(B.acc)[k];
     {
 #pragma clang fp contract(off)
         const double h = 0.5 * P.dt_2;
@@ -36,29 +36,29 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
         v.y = v.y + hd * a.y;
         v.z = v.z + hd * a.z;
     }
-    B.pos[par][k] = r;
-    B.vel[par][k] = v;
+    CUR(B.pos)[k] = r;
+    CUR(B.vel)[k] = v;
     if (!isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z)) atomicAdd(&B.flags[0], 1);
     if (r.z < 0.0 || r.z > P.L[2]) atomicAdd(&B.flags[1], 1);
     if (P.skin_mode) {
-        const double4 b = B.posb[par][k];
+        const double4 b = CUR(B.posb)[k];
         double dx = r.x - b.x, dy = r.y - b.y, dz = r.z - b.z;
         dx -= P.L[0] * rint(dx * P.invL[0]);
         dy -= P.L[1] * rint(dy * P.invL[1]);
         if (!(dx * dx + dy * dy + dz * dz <= P.skin_half_2)) atomicOr(B.rebuild, 1);
     }
     if (MIXED) {
-        const int c = B.cell_of[par][k] - P.cell_base;
+        const int c = CUR(B.cell_of)[k] - P.cell_base;
         if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) return;
         const int nx = P.nc[0], ny = P.nc[1];
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
         if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
         if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
-        float4 pf = B.posf[par][k];
+        float4 pf = CUR(B.posf)[k];
         pf.x = (float)rx; pf.y = (float)ry; pf.z = (float)rz;
-        B.posf[par][k] = pf;
-        B.velf[par][k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+        CUR(B.posf)[k] = pf;
+        CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
     }
 }
 
@@ -135,7 +135,7 @@ __device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
     if (!guard(B, in_range(gorig, B.n_beads), G_ORIG, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
     const int k = B.where[gorig];
     if (!guard(B, in_range(k, B.n_slots), G_WHERE, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
-    return B.pos[par][k];
+    return CUR(B.pos)[k];
 }
 
 // Bending contributions to the bead at chain position m (0-based) of a chain
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     constexpr bool FUSED = (MODE & M_FUSED) != 0;
     double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
     if (active) {
-        const double4 ri = B.pos[par][k];
+        const double4 ri = CUR(B.pos)[k];
         const int ti = wtype(ri.w);
         const int g = worig(ri.w);
         const int o = g - P.part_base;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
                 double *fh = f_heads + 3 * (size_t)(P.head_base + ol / P.n_mon);
                 fh[0] = F.x; fh[1] = F.y; fh[2] = F.z;
             }
-            double4 v = B.vel[par][k];
+            double4 v = CUR(B.vel)[k];
             double4 a;
             const double im = P.inv_mass_type[(ti - 1) & 3];
             {
@@ -262,9 +262,9 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
                 F.x = F.y = F.z = 0.0;
             }
             ke = P.mass_type[(ti - 1) & 3] * (v.x * v.x + v.y * v.y + v.z * v.z);
-            B.vel[par][k] = v;
-            B.acc[par][k] = a;
-            if (MIXED) B.velf[par][k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+            CUR(B.vel)[k] = v;
+            CUR(B.acc)[k] = a;
+            if (MIXED) CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
         f4.x = F.x; f4.y = F.y; f4.z = F.z;
         B.frc[k] = f4;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, con
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const SysDev &P = sys[blk_sys[k / BLK]];
         if (k - P.slot_base >= P.n_part) continue;
-        const double4 r = B.pos[par][k];
+        const double4 r = CUR(B.pos)[k];
         const int c = cell_of_pos(P, r.x, r.y, r.z);
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         bin_cell[k] = c;
@@ -402,13 +402,13 @@ __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, co
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         const int2 rg = crange[c];
         if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= n_slots, G_RANGE, k)) continue;
-        const int key = worig(B.pos[par][k].w);
+        const int key = worig(CUR(B.pos)[k].w);
         int rank = 0;
         bool ok = true;
         for (int m = rg.x; m < rg.y; ++m) {
             const int q = perm_tmp[m];
             if (!guard(B, in_range(q, n_slots), G_PERM, m)) { ok = false; break; }
-            rank += worig(B.pos[par][q].w) < key ? 1 : 0;
+            rank += worig(CUR(B.pos)[q].w) < key ? 1 : 0;
         }
         if (ok && guard(B, rank < rg.y - rg.x, G_RANGE, k)) perm[rg.x + rank] = k;
     }
@@ -420,21 +420,20 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
                                                 int n_slots) {
     if (!*B.rebuild) return;
     MFA_CUR(B);
-    const int nw = par ^ 1;
     for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
         const SysDev &P = sys[blk_sys[m / BLK]];
         if (m - P.slot_base >= P.n_part) continue;
         const int old = perm[m];
         if (!guard(B, in_range(old, n_slots), G_PERM, m)) continue;
-        const double4 r = B.pos[par][old];
+        const double4 r = CUR(B.pos)[old];
         if (!guard(B, in_range(worig(r.w), B.n_beads), G_ORIG, m)) continue;
-        const double4 v = B.vel[par][old];
-        B.pos[nw][m] = r;
-        B.vel[nw][m] = v;
-        B.acc[nw][m] = B.acc[par][old];
-        B.posb[nw][m] = r;
+        const double4 v = CUR(B.vel)[old];
+        NXT(B.pos)[m] = r;
+        NXT(B.vel)[m] = v;
+        NXT(B.acc)[m] = CUR(B.acc)[old];
+        NXT(B.posb)[m] = r;
         const int cg = bin_cell[old];
-        B.cell_of[nw][m] = cg;
+        NXT(B.cell_of)[m] = cg;
         B.where[worig(r.w)] = m;
         if (MIXED) {
             const int c = cg - P.cell_base;
@@ -444,8 +443,8 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
             double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
             if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
             if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
-            B.posf[nw][m] = make_float4((float)rx, (float)ry, (float)rz, mkwf(worig(r.w), wtype(r.w)));
-            B.velf[nw][m] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+            NXT(B.posf)[m] = make_float4((float)rx, (float)ry, (float)rz, mkwf(worig(r.w), wtype(r.w)));
+            NXT(B.velf)[m] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
     }
 }
@@ -468,23 +467,24 @@ __global__ __launch_bounds__(BLK) void k_upload(const SysDev *__restrict__ sys, 
     if (i >= P.n_part) return;
     const int k = P.slot_base + i;
     const int g = P.part_base + i;
-    B.pos[par][k] = make_double4(r0[3 * i], r0[3 * i + 1], r0[3 * i + 2], mkw(g, type[i]));
-    B.vel[par][k] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
-    B.acc[par][k] = a ? make_double4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0) : make_double4(0.0, 0.0, 0.0, 0.0);
+    CUR(B.pos)[k] = make_double4(r0[3 * i], r0[3 * i + 1], r0[3 * i + 2], mkw(g, type[i]));
+    CUR(B.vel)[k] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
+    CUR(B.acc)[k] = a ? make_double4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0) : make_double4(0.0, 0.0, 0.0, 0.0);
     B.frc[k] = make_double4(0.0, 0.0, 0.0, 0.0);
-    B.cell_of[par][k] = -1;
+    CUR(B.cell_of)[k] = -1;
     B.where[g] = k;
 }
 
-__global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys, int s, Bufs B, double *__restrict__ out,
-                                                 int which) {
+template <int WHICH>
+__global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys, int s, Bufs B, double *__restrict__ out) {
     MFA_CUR(B);
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= P.n_part) return;
     const int k = B.where[P.part_base + i];
     if (!guard(B, in_range(k, B.n_slots), G_WHERE, i)) return;
-    double4 q = which == 0 ? B.pos[par][k] : which == 1 ? B.vel[par][k] : which == 2 ? B.acc[par][k] : B.frc[k];
+    const double4 *src = WHICH == 0 ? CUR(B.pos) : WHICH == 1 ? CUR(B.vel) : WHICH == 2 ? CUR(B.acc) : B.frc;
+    const double4 q = src[k];
     out[3 * i] = q.x; out[3 * i + 1] = q.y; out[3 * i + 2] = q.z;
 }
 
@@ -563,8 +563,8 @@ __global__ __launch_bounds__(BLK) void k_prof_bin(const SysDev *__restrict__ sys
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
     if (k - P.slot_base >= P.n_part) return;
-    const double4 r = B.pos[par][k];
-    const double4 v = B.vel[par][k];
+    const double4 r = CUR(B.pos)[k];
+    const double4 v = CUR(B.vel)[k];
     const int o = worig(r.w) - P.part_base;
     const int sp = o < P.pd ? 0 : 1;
     const double inv_step = 1.0 / (P.L[2] / (double)nb);

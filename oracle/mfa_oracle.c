@@ -689,8 +689,9 @@ void orc_verlet_positions(const orc_params *p, double *r0, double *v, const doub
 }
 
 /* verlet_velocities, verlet_velocities.f90:14-153: f_on_heads snapshot,
- * a = F/m, second half-kick, heads frozen, kinetic energy t_fluid. */
-double orc_verlet_velocities(const orc_params *p, const double *force_in, const double *inv_mass,
+ * a = F/m, second half-kick, heads frozen (velocity, acceleration and force
+ * of every head zeroed, :95-119), kinetic energy t_fluid. */
+double orc_verlet_velocities(const orc_params *p, double *force_in, const double *inv_mass,
                              const double *mass, double *v, double *a, double *f_on_heads) {
     const int n_head_span = p->n_mon * p->n_chain;
     if (f_on_heads)
@@ -711,7 +712,10 @@ double orc_verlet_velocities(const orc_params *p, const double *force_in, const 
             v[3 * h + 0] = s * p->head_vel[0]; v[3 * h + 1] = s * p->head_vel[1];
         }
         v[3 * h + 2] = 0.0;
-        for (int d = 0; d < 3; ++d) a[3 * h + d] = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            force_in[3 * h + d] = 0.0;
+            a[3 * h + d] = 0.0;
+        }
     }
     double t = 0.0;
     for (int i = 0; i < p->n_part; ++i) {

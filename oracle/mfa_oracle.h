@@ -99,7 +99,7 @@ void orc_constant_force(const orc_params *p, const double *mass, const int *a_ty
 void orc_verlet_positions(const orc_params *p, double *r0, double *v, const double *a,
                           int relink, const int n_cells[3], const double inv_l_cell[3],
                           int *part_in_cell, int *r_nei, int *l_nei);
-double orc_verlet_velocities(const orc_params *p, const double *force_in, const double *inv_mass,
+double orc_verlet_velocities(const orc_params *p, double *force_in, const double *inv_mass,
                              const double *mass, double *v, double *a, double *f_on_heads);
 
 /* ---- observables (functions.f90 histo / velocity_prof mode 2) ---- */

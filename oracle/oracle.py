@@ -237,8 +237,8 @@ class Oracle:
         mass, inv = self.p.masses(a_type)
         v = np.array(v, float); a = np.zeros_like(v)
         fh = np.zeros((max(self.p.n_chain, 1), 3))
-        t = self.lib.orc_verlet_velocities(self.po, _p(np.ascontiguousarray(force, float)), _p(inv),
-                                           _p(mass), _p(v), _p(a), _p(fh))
+        force = np.array(force, float)          # heads' force is zeroed in this copy
+        t = self.lib.orc_verlet_velocities(self.po, _p(force), _p(inv), _p(mass), _p(v), _p(a), _p(fh))
         return v, a, t, fh
 
     def histo(self, r0, hist_dim=200):

@@ -89,6 +89,11 @@ def test_md_steps_match_reference(ref_mod):
     ro, vo, ao, fo, eo, _ = ref_mod.Oracle(p).md_steps(r0, v, np.zeros_like(r0), t, 3, 20, noise=0)
     assert np.abs(st["r0"] - ro).max() < 1e-12
     assert np.abs(st["v"] - vo).max() < 1e-11
+    assert np.abs(st["a"] - ao).max() < 1e-9
+    # the force array after the step, heads zeroed by verlet_velocities
+    assert np.abs(st["force"] - fo).max() <= 1e-10 * np.abs(fo).max()
+    heads = np.arange(p.n_chain) * p.n_mon
+    assert not st["force"][heads].any() and not fo[heads].any()
     for k in ("v_fluid_fluid", "v_intra_molec", "v_fluid_wall", "v_bend", "t_fluid"):
         assert eo[k] == pytest.approx(er[k], rel=1e-12), k
 
