@@ -29,6 +29,10 @@ HBM_PEAK = 8.0e12
 
 
 def run_gpu(precision, steps, warmup, p, t, r0, v):
+    """Warm-up steps (graph replay), then the timed region: `steps` steps with
+    HIP events around every kernel group on the library's stream (so the
+    per-kernel average durations come from the same steps as the wall time),
+    then the same number of steps as graph replays without events."""
     from mfa_brush_amd.sim import Simulation
     s = Simulation(p, precision=precision)
     s.set_state(r0, v, t)
@@ -39,15 +43,45 @@ def run_gpu(precision, steps, warmup, p, t, r0, v):
     s.md_steps(steps)
     s.synchronize()
     dt = time.perf_counter() - t0
-    pair_ms, pair_n = s.kernel_time("pair")
     tot = {k: s.kernel_time(k) for k in ("vv1", "rebuild", "pair", "bonded")}
     s.set_timing(False)
+    t0 = time.perf_counter()
+    s.md_steps(steps, graph=True)
+    s.synchronize()
+    dt_graph = time.perf_counter() - t0
     npairs = s.pair_count()
     fl = s.flags()
     e = s.energies()
     s.close()
-    return dict(seconds=dt, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl, energies=e,
-                kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+    pair_ms, pair_n = tot["pair"]
+    return dict(seconds=dt, seconds_graph=dt_graph, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl,
+                energies=e, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+
+
+def traffic_from_profile():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's reading), or
+    None when no such summary is present."""
+    f = os.path.join(ROOT, "profiles", "k_pair32_c2_traffic.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def calib_copy(mib):
+    """A plain device-to-device copy of `mib` MiB (read + write of that many
+    bytes), the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("calib copy: %d MiB x 3" % mib, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(p, t, r0, v, n_steps):
@@ -73,12 +107,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU: there is no multi-GPU path (a channel would shard into x-slabs "
+              "with a halo exchange; replicas shard by replica)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib)
     from mfa_brush_amd import configs
     p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=0.4)
     n = p.n_part
@@ -111,15 +149,18 @@ def main():
                                "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005, skin 0.4",
                    "n_beads": n, "interacting_pairs": mixed["npairs"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile(),
                      "kernel": "k_pair32 (VALU FP32; no MFMA on the pair path)",
                      "flops_per_launch": FLOPS_PER_PAIR * mixed["npairs"], "avg_launch_us": pair_avg_s * 1e6},
         "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in mixed["kernels"].items()},
+        "graph_ms_per_step": 1e3 * mixed["seconds_graph"] / args.steps,
+        "rebuilds": mixed["flags"]["rebuilds"],
     }
     if fp64 is not None:
         v64 = n * args.steps / fp64["seconds"]
         p64 = fp64["pair_ms"] / max(fp64["pair_n"], 1) / 1e3
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+                       "graph_ms_per_step": 1e3 * fp64["seconds_graph"] / args.steps,
                        "pair_avg_us": p64 * 1e6,
                        "pair_frac_fp64_peak": FLOPS_PER_PAIR * fp64["npairs"] / p64 / PEAK_FP64,
                        "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in fp64["kernels"].items()}}

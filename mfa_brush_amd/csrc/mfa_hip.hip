@@ -174,11 +174,8 @@ static int to_device(mfa_ctx *c, void *dst, const void *src, size_t bytes, const
 
 static void free_all(mfa_ctx *c) {
     auto F = [](void *p) { if (p) (void)hipFree(p); };
-    for (int h = 0; h < 2; ++h) {
-        F(c->B.pos[h]); F(c->B.vel[h]); F(c->B.acc[h]); F(c->B.posb[h]);
-        F(c->B.posf[h]); F(c->B.velf[h]); F(c->B.cell_of[h]);
-    }
-    F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.flags); F(c->B.t_idx);
+    F(c->B.pos); F(c->B.vel); F(c->B.acc); F(c->B.posb); F(c->B.posf); F(c->B.velf); F(c->B.cell_of);
+    F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.chan_rb); F(c->B.flags); F(c->B.t_idx);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_perm_tmp); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
@@ -445,12 +442,12 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     if (hipHostMalloc((void **)&c->h_stage, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
         c->h_stage = nullptr; free_all(c); delete c; return fail(MFA_E_HIP, "pinned staging allocation failed");
     }
-    for (int h = 0; h < 2; ++h) {
-        ALLOC(c->B.pos[h], c->S_tot); ALLOC(c->B.vel[h], c->S_tot); ALLOC(c->B.acc[h], c->S_tot);
-        ALLOC(c->B.posb[h], c->S_tot); ALLOC(c->B.cell_of[h], c->S_tot);
-        if (precision == MFA_PREC_MIXED) { ALLOC(c->B.posf[h], c->S_tot); ALLOC(c->B.velf[h], c->S_tot); }
-    }
+    // two halves of S_tot slots each (current / next, selected by parity)
+    ALLOC(c->B.pos, 2 * (size_t)c->S_tot); ALLOC(c->B.vel, 2 * (size_t)c->S_tot); ALLOC(c->B.acc, 2 * (size_t)c->S_tot);
+    ALLOC(c->B.posb, 2 * (size_t)c->S_tot); ALLOC(c->B.cell_of, 2 * (size_t)c->S_tot);
+    if (precision == MFA_PREC_MIXED) { ALLOC(c->B.posf, 2 * (size_t)c->S_tot); ALLOC(c->B.velf, 2 * (size_t)c->S_tot); }
     ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, 1); ALLOC(c->B.rebuild, 1);
+    ALLOC(c->B.chan_rb, n_sys);
     ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1);
     ALLOC(c->d_sys, n_sys); ALLOC(c->d_blk_sys, c->n_blocks); ALLOC(c->d_cell_sys, c->C_tot);
     ALLOC(c->d_cell_count, c->C_tot); ALLOC(c->d_pre, c->C_tot);
@@ -504,7 +501,7 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
     if (!rc) {
         LAUNCH(k_upload, (int)((n + BLK - 1) / BLK), c->st, c->d_sys, isys, c->B, (const double *)dr, (const double *)dv,
                (const double *)da, (const int *)c->d_istage);
-        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 1);
+        hipLaunchKernelGGL(k_mark_rebuild, dim3(1), dim3(1), 0, c->st, c->B, (int)isys);
         rc = sync_for(c, "state upload");
     }
     (void)hipFree(dr); (void)hipFree(dv); if (da) (void)hipFree(da);
@@ -520,7 +517,7 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
 static int ensure_cells(mfa_ctx *c) {
     // A rebuild is pending after set_state; run it (it is a no-op otherwise).
     enqueue_rebuild(c);
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 0);
+    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
     return 0;
 }
 
@@ -565,7 +562,7 @@ int mfa_hip_verlet_positions(mfa_ctx *c) {
     ensure_cells(c);
     enqueue_vv1(c);
     enqueue_rebuild(c);
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, c->st, c->B.rebuild, 0);
+    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -779,7 +776,8 @@ int mfa_hip_get_cells(mfa_ctx *c, int32_t isys, int32_t *n_cells, int32_t *cell_
     int par = 0;
     if (int rc = to_host(c, &par, c->B.parity, sizeof(int), "parity")) return rc;
     std::vector<double4> pos(d.n_part);
-    if (int rc = to_host(c, pos.data(), c->B.pos[par] + d.slot_base, d.n_part * sizeof(double4), "positions")) return rc;
+    const double4 *cur = c->B.pos + (size_t)(par & 1) * (size_t)c->B.n_slots;
+    if (int rc = to_host(c, pos.data(), cur + d.slot_base, d.n_part * sizeof(double4), "positions")) return rc;
     for (int q = 0; q < d.n_cells_tot; ++q) {
         if (cell_start) cell_start[q] = rg[q].x - d.slot_base;
         if (cell_count) cell_count[q] = rg[q].y - rg[q].x;

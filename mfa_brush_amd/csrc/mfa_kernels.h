@@ -46,27 +46,34 @@ struct SysDev {
 
 // Double-buffered bead arrays: a rebuild permutes into the other half and a
 // one-thread kernel flips `parity`, so no copy-back pass is needed and the
-// host never has to know whether the device decided to rebuild.
+// host never has to know whether the device decided to rebuild.  Each array
+// is one allocation of 2 * n_slots entries; the current half starts at
+// parity * n_slots.  The halves are reached by offset arithmetic on a single
+// pointer, never by choosing between pointer fields of this struct at run
+// time: the AMDGPU backend of this toolchain left the pointer register
+// undefined on one branch of a run-time choice between pointers indexed
+// dynamically out of the by-value kernel-argument struct.
 struct Bufs {
-    double4 *pos[2], *vel[2], *acc[2], *posb[2];
-    float4 *posf[2], *velf[2];
-    int *cell_of[2];
+    double4 *pos, *vel, *acc, *posb;
+    float4 *posf, *velf;
+    int *cell_of;
     double4 *frc;
     int *where;
     int *parity;
-    int *rebuild;
+    int *rebuild;       // any channel rebuilds this step (gates the rebuild kernels)
+    int *chan_rb;       // per channel: this channel's cells are rebuilt this step
     int *flags;         // [0..3] anomaly counters, [6] offending index, [7] first guard code
     long long *t_idx;
     int n_slots, n_beads, n_cells, n_heads;   // buffer extents, for the guards
 };
 
-#define MFA_CUR(B) const int par = *(B).parity; (void)par
-// The two halves are selected with constant indices (no dynamic indexing
-// into the by-value kernel-argument struct): the AMDGPU backend of this
-// toolchain left the pointer register undefined on one branch of a runtime
-// switch over such dynamically indexed kernel-argument pointers.
-#define CUR(arr) (par ? (arr)[1] : (arr)[0])
-#define NXT(arr) (par ? (arr)[0] : (arr)[1])
+#define MFA_CUR(B)                                                        \
+    const int par = *(B).parity & 1;                                      \
+    const size_t cur_off = (size_t)par * (size_t)(B).n_slots;             \
+    const size_t nxt_off = (size_t)(par ^ 1) * (size_t)(B).n_slots;       \
+    (void)cur_off; (void)nxt_off
+#define CUR(arr) ((arr) + cur_off)
+#define NXT(arr) ((arr) + nxt_off)
 
 // Device-side guards.  Every data-dependent index (cell ids, cell ranges,
 // permutations, bead identities, chain neighbours) is checked against the

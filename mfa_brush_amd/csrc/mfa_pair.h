@@ -130,6 +130,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     const SysDev &P = sys[s];
     load_tables(P, tab);
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
+    if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
     __syncthreads();
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
+    if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
     __syncthreads();
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;

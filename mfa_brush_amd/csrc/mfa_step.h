@@ -18,7 +18,7 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
-    if (threadIdx.x == 0 && P.skin_mode == 0) *B.rebuild = 1;
+    if (threadIdx.x == 0 && P.skin_mode == 0) { *B.rebuild = 1; B.chan_rb[s] = 1; }
     if (k - P.slot_base >= P.n_part) return;
     double4 r = CUR(B.pos)[k];
     double4 v = CUR(B.vel)[k];
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
         double dx = r.x - b.x, dy = r.y - b.y, dz = r.z - b.z;
         dx -= P.L[0] * rint(dx * P.invL[0]);
         dy -= P.L[1] * rint(dy * P.invL[1]);
-        if (!(dx * dx + dy * dy + dz * dz <= P.skin_half_2)) atomicOr(B.rebuild, 1);
+        if (!(dx * dx + dy * dy + dz * dz <= P.skin_half_2)) { atomicOr(B.rebuild, 1); atomicOr(&B.chan_rb[s], 1); }
     }
     if (MIXED) {
         const int c = CUR(B.cell_of)[k] - P.cell_base;
@@ -131,7 +131,7 @@ __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm
 
 enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64 };
 
-__device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
+__device__ __forceinline__ double4 bead_at(const Bufs &B, size_t cur_off, int gorig) {
     if (!guard(B, in_range(gorig, B.n_beads), G_ORIG, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
     const int k = B.where[gorig];
     if (!guard(B, in_range(k, B.n_slots), G_WHERE, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
@@ -141,14 +141,14 @@ __device__ __forceinline__ double4 bead_at(const Bufs &B, int par, int gorig) {
 // Bending contributions to the bead at chain position m (0-based) of a chain
 // of nm beads starting at global index g0, in the reference's loop order
 // (centres ascending): +fn of centre m-1, -fp-fn of centre m, +fp of m+1.
-__device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, int par, int g0, int nm, int m,
+__device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, size_t cur_off, int g0, int nm, int m,
                                           const double4 ri, double3 &F, double &ev) {
     // positions of the beads m-2 .. m+2 of the chain (absent ones are never used)
     double4 rm2 = ri, rm1 = ri, rp1 = ri, rp2 = ri;
-    if (m >= 2) rm2 = bead_at(B, par, g0 + m - 2);
-    if (m >= 1) rm1 = bead_at(B, par, g0 + m - 1);
-    if (m + 1 < nm) rp1 = bead_at(B, par, g0 + m + 1);
-    if (m + 2 < nm) rp2 = bead_at(B, par, g0 + m + 2);
+    if (m >= 2) rm2 = bead_at(B, cur_off, g0 + m - 2);
+    if (m >= 1) rm1 = bead_at(B, cur_off, g0 + m - 1);
+    if (m + 1 < nm) rp1 = bead_at(B, cur_off, g0 + m + 1);
+    if (m + 2 < nm) rp2 = bead_at(B, cur_off, g0 + m + 2);
     if (m - 1 >= 1 && m - 1 <= nm - 2) {
         BendOut o = bend_angle(P, rm2, rm1, ri);
         F.x += o.fn[0]; F.y += o.fn[1]; F.z += o.fn[2];
@@ -212,21 +212,21 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         if (MODE & (M_FENE | M_FUSED)) {
             int str = 0;
             if (m > 0) {
-                BondOut b = fene(P, bead_at(B, par, g - 1), ri, &str);
+                BondOut b = fene(P, bead_at(B, cur_off, g - 1), ri, &str);
                 F.x += b.fx; F.y += b.fy; F.z += b.fz;
             }
             if (m < nm - 1) {
-                BondOut b = fene(P, ri, bead_at(B, par, g + 1), &str);
+                BondOut b = fene(P, ri, bead_at(B, cur_off, g + 1), &str);
                 F.x -= b.fx; F.y -= b.fy; F.z -= b.fz;
                 e_fene += b.pot;
             }
             if (str) atomicAdd(&B.flags[2], 1);
         }
         if ((MODE & M_BEND) || (FUSED && P.bending)) {
-            if (brush) bend_bead(P, B, par, g0, nm, m, ri, F, e_bend);
+            if (brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bend);
         }
         if ((MODE & M_BENDM) || (FUSED && P.bending_melt)) {
-            if (!brush) bend_bead(P, B, par, g0, nm, m, ri, F, e_bendm);
+            if (!brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bendm);
         }
         if ((MODE & M_CONST) || (FUSED && P.poiseuille)) {
             if (ti != 1) {
@@ -299,7 +299,17 @@ __global__ __launch_bounds__(BLK) void k_reduce(const SysDev *__restrict__ sys, 
 // per-bead relinking update_part_cell, cell_list.f90:215-268, and
 // neigh_list, :56-106).  Every kernel is gated on *rebuild.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLK) void k_set_flag(int *flag, int v) { *flag = v; }
+// Request a rebuild of channel s (after its state was uploaded).
+__global__ __launch_bounds__(BLK) void k_mark_rebuild(Bufs B, int s) {
+    *B.rebuild = 1;
+    B.chan_rb[s] = 1;
+}
+
+// Clear the rebuild requests of all n_sys channels.
+__global__ __launch_bounds__(BLK) void k_clear_rebuild(Bufs B, int n_sys) {
+    for (int s = threadIdx.x; s < n_sys; s += BLK) B.chan_rb[s] = 0;
+    if (threadIdx.x == 0) *B.rebuild = 0;
+}
 
 __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                             int *__restrict__ cell_count, int *__restrict__ bin_cell,
@@ -307,10 +317,19 @@ __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, con
     if (!*B.rebuild) return;
     MFA_CUR(B);
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
-        const SysDev &P = sys[blk_sys[k / BLK]];
+        const int si = blk_sys[k / BLK];
+        const SysDev &P = sys[si];
         if (k - P.slot_base >= P.n_part) continue;
-        const double4 r = CUR(B.pos)[k];
-        const int c = cell_of_pos(P, r.x, r.y, r.z);
+        // A channel that does not rebuild is binned into its build-time cells:
+        // with the index order inside a cell this reproduces its layout
+        // exactly, so its results do not depend on the other channels.
+        int c;
+        if (B.chan_rb[si]) {
+            const double4 r = CUR(B.pos)[k];
+            c = cell_of_pos(P, r.x, r.y, r.z);
+        } else {
+            c = CUR(B.cell_of)[k];
+        }
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         bin_cell[k] = c;
         bin_rank[k] = atomicAdd(&cell_count[c], 1);
@@ -421,7 +440,8 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
     if (!*B.rebuild) return;
     MFA_CUR(B);
     for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
-        const SysDev &P = sys[blk_sys[m / BLK]];
+        const int si = blk_sys[m / BLK];
+        const SysDev &P = sys[si];
         if (m - P.slot_base >= P.n_part) continue;
         const int old = perm[m];
         if (!guard(B, in_range(old, n_slots), G_PERM, m)) continue;
@@ -431,7 +451,9 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         NXT(B.pos)[m] = r;
         NXT(B.vel)[m] = v;
         NXT(B.acc)[m] = CUR(B.acc)[old];
-        NXT(B.posb)[m] = r;
+        double4 rb = r;                                 // skin reference of the last build
+        if (!B.chan_rb[si]) rb = CUR(B.posb)[old];
+        NXT(B.posb)[m] = rb;
         const int cg = bin_cell[old];
         NXT(B.cell_of)[m] = cg;
         B.where[worig(r.w)] = m;
@@ -507,14 +529,14 @@ __global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, i
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
     const bool act = i < P.n_part;
-    double4 ri = act ? bead_at(B, par, P.part_base + i) : make_double4(0, 0, 0, 0);
+    double4 ri = act ? bead_at(B, cur_off, P.part_base + i) : make_double4(0, 0, 0, 0);
     const int ti = act ? wtype(ri.w) : 1;
     double e = 0.0;
     int cnt = 0;
     unsigned long long h = 0ull;
     for (int j0 = 0; j0 < P.n_part; j0 += BLK) {
         const int jj = j0 + threadIdx.x;
-        tile[threadIdx.x] = jj < P.n_part ? bead_at(B, par, P.part_base + jj) : make_double4(0, 0, 0, 0);
+        tile[threadIdx.x] = jj < P.n_part ? bead_at(B, cur_off, P.part_base + jj) : make_double4(0, 0, 0, 0);
         __syncthreads();
         const int jn = min(BLK, P.n_part - j0);
         if (act) {
