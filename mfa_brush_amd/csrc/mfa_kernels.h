@@ -132,6 +132,17 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 // Box-Muller normal from two 32-bit words with IEEE +,-,*,/,sqrt only (FP32,
 // correctly rounded divide and square root, no contraction), so the value is
 // the same bits on the host and on the device.
+//
+// The FP32 divide and square root are taken from FP64 and rounded once: the
+// HIP __fsqrt_rn of this toolchain is the native v_sqrt_f32 (not correctly
+// rounded) unless OCML_BASIC_ROUNDED_OPERATIONS is defined.  Rounding a
+// correctly rounded (or few-ulp) FP64 quotient or root to FP32 gives the
+// correctly rounded FP32 result: for a root, 4x - (2k+1)^2 is a nonzero odd
+// integer, so sqrt(x) lies ~2^24 FP64 ulps from any FP32 midpoint; for a
+// quotient, 53 >= 2*24+1 makes the double rounding innocuous.
+__device__ __forceinline__ float div_rn_f32(float a, float b) { return (float)((double)a / (double)b); }
+__device__ __forceinline__ float sqrt_rn_f32(float a) { return (float)sqrt((double)a); }
+
 __device__ __forceinline__ float gauss_from_bits(uint32_t x0, uint32_t x1) {
 #pragma clang fp contract(off)
     float u1 = (float)((x0 >> 8) + 1u) * 5.9604644775390625e-08f;
@@ -139,7 +150,7 @@ __device__ __forceinline__ float gauss_from_bits(uint32_t x0, uint32_t x1) {
     int e = (int)(b >> 23) - 127;
     float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
     if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
-    float s = __fdiv_rn(m - 1.0f, m + 1.0f);
+    float s = div_rn_f32(m - 1.0f, m + 1.0f);
     float s2 = s * s;
     float p = 0.222222222f;
     p = p * s2 + 0.285714286f;
@@ -147,7 +158,7 @@ __device__ __forceinline__ float gauss_from_bits(uint32_t x0, uint32_t x1) {
     p = p * s2 + 0.666666667f;
     p = p * s2 + 2.0f;
     float lnu = (float)e * 0.693147182f + s * p;
-    float rad = __fsqrt_rn(-2.0f * lnu);
+    float rad = sqrt_rn_f32(-2.0f * lnu);
     uint32_t q = x1 >> 8;
     uint32_t k = (q + (1u << 21)) >> 22;
     float f = (float)((int32_t)q - (int32_t)(k << 22)) * 2.38418579e-07f;

@@ -83,3 +83,25 @@ def test_channel_generators():
     assert dd.min() > 0.9
     ps, ts, rs, vs = configs.channel_scaled(12)
     assert abs(ps.n_part / np.prod(ps.boundary) - 0.8) < 0.01
+
+
+def test_fp32_divide_and_sqrt_from_fp64_are_correctly_rounded():
+    """The device Box-Muller takes its FP32 divide and square root from FP64
+    and rounds once (mfa_kernels.h div_rn_f32 / sqrt_rn_f32); the oracle uses
+    FP32 IEEE divide and sqrtf.  Both agree when double rounding is
+    innocuous, checked here exhaustively over the arguments Box-Muller can
+    see: every float m in [1, sqrt 2] for (m-1)/(m+1), and every float in
+    [2^-24, 64) for sqrt(-2 ln u)."""
+    lo = np.float32(1.0).view(np.uint32)
+    hi = np.float32(1.41421356).view(np.uint32)
+    m = np.arange(lo, hi + 1, dtype=np.uint32).view(np.float32)
+    a, b = m - np.float32(1), m + np.float32(1)
+    assert np.array_equal((a.astype(np.float64) / b.astype(np.float64)).astype(np.float32), a / b)
+    # half-open [0.5, 1): the (m-1)/(m+1) branch after m*0.5 is in this range too
+    m = np.arange(np.float32(0.70710677).view(np.uint32), np.float32(1.0).view(np.uint32),
+                  dtype=np.uint32).view(np.float32)
+    a, b = m - np.float32(1), m + np.float32(1)
+    assert np.array_equal((a.astype(np.float64) / b.astype(np.float64)).astype(np.float32), a / b)
+    for e in range(-24, 6):
+        x = (np.arange(1 << 23, dtype=np.uint32) | np.uint32((e + 127) << 23)).view(np.float32)
+        assert np.array_equal(np.sqrt(x.astype(np.float64)).astype(np.float32), np.sqrt(x)), e
