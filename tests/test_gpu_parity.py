@@ -256,22 +256,40 @@ def test_mixed_precision_force_error(oracle_mod):
     assert e == pytest.approx(eo, rel=1e-5)
 
 
-def test_mixed_precision_dpd_statistics():
-    """Thermostat on: the mixed path keeps kT=1 (fluctuation-dissipation)."""
-    p, t, r0, v = configs.channel_c1(thermostat=True, rc_dpd=1.0)
-    p = p.replace(skin=0.4)
-    with Simulation(p, precision="mixed") as s:
+def _mean_kt(p, t, r0, v, precision, equil=400, blocks=20, per=20):
+    with Simulation(p, precision=precision) as s:
         s.set_state(r0, v, t)
-        s.md_steps(400, graph=True)
+        s.md_steps(equil, graph=True)
         kes = []
-        for _ in range(20):
-            s.md_steps(20, graph=True)
+        for _ in range(blocks):
+            s.md_steps(per, graph=True)
             kes.append(s.energies()["t_fluid"])
         fl = s.flags()
-    n_free = p.n_part - p.n_chain
-    kT = 2.0 * np.mean(kes) / (3 * n_free)
-    assert abs(kT - 1.0) < 0.05, kT
     assert fl["nonfinite"] == 0
+    return 2.0 * np.mean(kes) / (3 * (p.n_part - p.n_chain))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_dpd_thermostat_holds_kt(precision):
+    """Fluctuation-dissipation: with DPD on every LJ pair (the reference's
+    default, no DPD_CUT_OFF) the channel holds kT = 1 within 5 %.  (The
+    oracle, run the same way on the CPU, gives 1.015.)"""
+    p, t, r0, v = configs.channel_c1(thermostat=True, rc_dpd=None)
+    p = p.replace(skin=0.4)
+    kT = _mean_kt(p, t, r0, v, precision)
+    assert abs(kT - 1.0) < 0.05, kT
+
+
+def test_mixed_precision_dpd_statistics():
+    """DPD_CUT_OFF 1.0 with heads excluded thermostats few pairs, and this
+    start keeps heating (the oracle reaches kT ~ 1.8 after 800 steps): the
+    mixed path must follow the FP64 path (which matches the oracle step by
+    step) statistically, within 5 % in mean kT."""
+    p, t, r0, v = configs.channel_c1(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    k32 = _mean_kt(p, t, r0, v, "mixed")
+    k64 = _mean_kt(p, t, r0, v, "fp64")
+    assert abs(k32 / k64 - 1.0) < 0.05, (k32, k64)
 
 
 def test_pair_set_scaled_c2(oracle_mod):
@@ -292,18 +310,29 @@ def test_pair_set_scaled_c2(oracle_mod):
 
 def test_full_c2_pair_count_and_momentum(oracle_mod):
     """Full C2 (998,784 beads): the kernel's interacting-pair count equals the
-    oracle's linked-cell count, and the pair forces sum to ~0."""
+    oracle's linked-cell count, and the pair forces sum to ~0.
+
+    FP64: each pair's two evaluations are exact negatives, so only the
+    per-bead summation rounds.  Mixed: each bead evaluates the pair in its own
+    cell frame in FP32, so the two evaluations differ by rounding; the sum is
+    a random walk of per-bead errors, bounded here by the stated per-bead
+    error budget (1e-5 of the largest force)."""
     p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0)
     p = p.replace(skin=0.4)
+    co, _ = oracle_mod.Oracle(p).pair_stats_cells(r0, t)
+    sums = {}
     for prec in ("fp64", "mixed"):
         with Simulation(p, precision=prec) as s:
             s.set_state(r0, v, t)
             s.fluid_fluid()
             npair = s.pair_count()
             f = s.get_state()["force"]
-        co, _ = _oracle(oracle_mod, p).pair_stats_cells(r0, t)
-        assert npair == co.sum() // 2
-        assert np.abs(f.sum(0)).max() < 1e-6 * np.abs(f).max() * math.sqrt(len(f))
+        assert npair == co.sum() // 2, prec
+        sums[prec] = (np.abs(f.sum(0)).max(), np.abs(f).max())
+    print("C2 |sum F|, max|F|:", sums)
+    n = math.sqrt(p.n_part)
+    assert sums["fp64"][0] < 1e-11 * sums["fp64"][1] * n, sums
+    assert sums["mixed"][0] < 1e-5 * sums["mixed"][1] * n, sums
 
 
 def test_profiles_vs_oracle(oracle_mod):
