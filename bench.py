@@ -4,7 +4,7 @@ Workload: the 998,784-bead brush channel (box 102*sqrt2 x 102*sqrt2 x 60,
 2 x 102^2 grafted chains of 32 at grafting density 0.5, 332,928 solvent,
 LJ rc 2.5 / brush-solvent WCA, DPD gamma 4.5 rc 1, kT 1, dt 0.005, 9-3
 walls), synthetic initial state (seeded), DPD thermostat on, cells rebuilt
-on the skin criterion (skin 0.4).  A step is one full md_main iteration:
+on the skin criterion (skin 0.4 by default, --skin).  A step is one full md_main iteration:
 integrate, cell maintenance, LJ+DPD pairs, bonds, walls, second kick.
 
 value = beads x steps / seconds of the mixed-precision path (FP32 pair
@@ -28,13 +28,14 @@ PEAK_FP64 = 78.6e12          # MI355X vector FP64
 HBM_PEAK = 8.0e12
 
 
-def run_gpu(precision, steps, warmup, p, t, r0, v):
+def run_gpu(precision, steps, warmup, p, t, r0, v, nlist=-1):
     """Warm-up steps (graph replay), then the timed region: `steps` steps with
     HIP events around every kernel group on the library's stream (so the
     per-kernel average durations come from the same steps as the wall time),
     then the same number of steps as graph replays without events."""
     from mfa_brush_amd.sim import Simulation
     s = Simulation(p, precision=precision)
+    s.set_neighbor_list(nlist)
     s.set_state(r0, v, t)
     s.md_steps(warmup, graph=True)
     s.synchronize()
@@ -51,11 +52,12 @@ def run_gpu(precision, steps, warmup, p, t, r0, v):
     dt_graph = time.perf_counter() - t0
     npairs = s.pair_count()
     fl = s.flags()
+    cap = s.neighbor_list()
     e = s.energies()
     s.close()
     pair_ms, pair_n = tot["pair"]
     return dict(seconds=dt, seconds_graph=dt_graph, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl,
-                energies=e, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+                energies=e, list_cap=cap, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
 
 
 def traffic_from_profile():
@@ -107,6 +109,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-list", action="store_true", help="pair kernels scan the cells every step")
+    ap.add_argument("--skin", type=float, default=0.4,
+                    help="Verlet skin (cells and lists); the pair set does not depend on it")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -118,15 +123,16 @@ def main():
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib)
     from mfa_brush_amd import configs
-    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=0.4)
+    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=args.skin)
     n = p.n_part
     print("workload: %d beads, box %s" % (n, p.boundary), file=sys.stderr, flush=True)
 
-    mixed = run_gpu("mixed", args.steps, args.warmup, p, t, r0, v)
+    nlist = 0 if args.no_list else -1
+    mixed = run_gpu("mixed", args.steps, args.warmup, p, t, r0, v, nlist)
     print("mixed: %.3f s for %d steps" % (mixed["seconds"], args.steps), file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_gpu("fp64", args.steps, args.warmup, p, t, r0, v)
+        fp64 = run_gpu("fp64", args.steps, args.warmup, p, t, r0, v, nlist)
         print("fp64: %.3f s for %d steps" % (fp64["seconds"], args.steps), file=sys.stderr, flush=True)
 
     value = n * args.steps / mixed["seconds"]
@@ -146,7 +152,8 @@ def main():
         "dtype": "fp32-pair/fp64-integrate",
         "data": "synthetic (seeded column-lattice start, Maxwell-Boltzmann velocities)",
         "config": {"workload": "brush channel 998,784 beads, box 144.25x144.25x60, 2x10404 chains N=32, "
-                               "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005, skin 0.4",
+                               "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005",
+                   "skin": args.skin,
                    "n_beads": n, "interacting_pairs": mixed["npairs"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile(),
@@ -155,6 +162,7 @@ def main():
         "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in mixed["kernels"].items()},
         "graph_ms_per_step": 1e3 * mixed["seconds_graph"] / args.steps,
         "rebuilds": mixed["flags"]["rebuilds"],
+        "neighbor_list": {"capacity": mixed["list_cap"], "overflowed_beads": mixed["flags"]["list_overflow"]},
     }
     if fp64 is not None:
         v64 = n * args.steps / fp64["seconds"]

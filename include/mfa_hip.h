@@ -88,6 +88,7 @@ typedef struct mfa_flags {
     int32_t out_of_box_z;    /* beads with z outside [0, Lz] (fluid_wall.f90:86,106 warnings) */
     int32_t fene_stretched;  /* bonds clamped at 0.98 R0^2 (intra_molec.f90:39-43) */
     int32_t rebuilds;        /* cell-list rebuilds since creation */
+    int32_t list_overflow;   /* beads whose Verlet list exceeded the capacity (they scanned the cells) */
 } mfa_flags;
 
 typedef struct mfa_ctx mfa_ctx;
@@ -159,6 +160,16 @@ int  mfa_hip_get_cells(mfa_ctx *ctx, int32_t isys, int32_t *n_cells, int32_t *ce
 int  mfa_hip_set_timing(mfa_ctx *ctx, int32_t on);
 int  mfa_hip_get_kernel_time(mfa_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
 int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off) */
+
+/* Verlet-list capacity (entries per bead) of the pair kernels: -1 automatic
+ * (default), 0 no list (the pair kernels scan the 27 or 125 neighbour cells
+ * every step), > 0 explicit.  The list holds the partners within the
+ * interaction range plus the skin; it is rebuilt with the cells, so the pair
+ * set and the forces are the same bits with or without it.  No counterpart in
+ * the reference (its BIN_TYPE 0/1 Verlet lists are host data structures,
+ * check_skin.f90 / binning.f90); a tuning knob of this implementation. */
+int  mfa_hip_set_neighbor_list(mfa_ctx *ctx, int32_t capacity);
+int  mfa_hip_get_neighbor_list(mfa_ctx *ctx, int32_t *capacity);
 int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* interacting pairs, last evaluation */
 
 const char *mfa_hip_last_error(void);

@@ -49,7 +49,7 @@ class Energies(C.Structure):
 
 class Flags(C.Structure):
     _fields_ = [("nonfinite", C.c_int32), ("out_of_box_z", C.c_int32), ("fene_stretched", C.c_int32),
-                ("rebuilds", C.c_int32)]
+                ("rebuilds", C.c_int32), ("list_overflow", C.c_int32)]
 
 
 # every symbol include/mfa_hip.h declares, with its ctypes signature
@@ -86,6 +86,8 @@ SIGNATURES = {
     "mfa_hip_set_timing": [_vp, _i32],
     "mfa_hip_get_kernel_time": [_vp, C.c_char_p, _vp, _vp],
     "mfa_hip_set_virial": [_vp, _i32],
+    "mfa_hip_set_neighbor_list": [_vp, _i32],
+    "mfa_hip_get_neighbor_list": [_vp, _vp],
     "mfa_hip_get_pair_count": [_vp, _i32, _vp],
     "mfa_hip_last_error": [],
     "mfa_hip_version": [],

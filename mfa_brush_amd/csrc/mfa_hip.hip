@@ -74,6 +74,7 @@ struct mfa_ctx {
     bool have_state = false;
     std::vector<char> uploaded;
     bool virial = false;
+    int nl_request = -1;          // Verlet-list capacity: -1 automatic, 0 off, > 0 entries per bead
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -176,6 +177,7 @@ static void free_all(mfa_ctx *c) {
     auto F = [](void *p) { if (p) (void)hipFree(p); };
     F(c->B.pos); F(c->B.vel); F(c->B.acc); F(c->B.posb); F(c->B.posf); F(c->B.velf); F(c->B.cell_of);
     F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.chan_rb); F(c->B.flags); F(c->B.t_idx);
+    F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_perm_tmp); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
@@ -255,6 +257,13 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
     d.head_vel[0] = p.head_vel[0]; d.head_vel[1] = p.head_vel[1];
     d.skin_mode = p.skin > 0.0 ? 1 : 0;
     d.skin_half_2 = 0.25 * p.skin * p.skin;
+    // Verlet-list radius per type pair: the LJ range, or the DPD range where
+    // the thermostat acts beyond it (DPD_CUT_OFF), plus the skin
+    for (int t = 0; t < 16; ++t) {
+        double r = p.range_2[t] > 0.0 ? sqrt(p.range_2[t]) : 0.0;
+        if (p.thermostat && p.dpd_cut_off) r = std::max(r, sqrt(p.r_cut_dpd_2));
+        d.rl2[t] = r > 0.0 ? (r + p.skin) * (r + p.skin) : -1.0;
+    }
     d.seed = p.seed;
     d.n_part = p.n_mon * p.n_chain + p.n_mon_d * p.n_chain_d;
     d.n_mon = p.n_mon; d.n_chain = p.n_chain; d.n_mon_d = p.n_mon_d; d.n_chain_d = p.n_chain_d;
@@ -271,6 +280,43 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
 // ---------------------------------------------------------------------------
 // step pieces
 // ---------------------------------------------------------------------------
+// Verlet-list storage for the requested capacity (nl_request: -1 automatic,
+// 0 off).  The automatic capacity covers twice the mean bead density of the
+// densest channel over the largest list sphere, plus a margin; beads with
+// longer lists scan the cells instead (counted in flags[4]).  Lists are used
+// only when some channel has a skin (otherwise every step rebuilds).
+static int alloc_list(mfa_ctx *c) {
+    if (c->B.nl) { (void)hipFree(c->B.nl); c->B.nl = nullptr; }
+    if (c->B.nl_n) { (void)hipFree(c->B.nl_n); c->B.nl_n = nullptr; }
+    c->B.nl_cap = 0;
+    int cap = c->nl_request;
+    if (cap < 0) {
+        bool any_skin = false;
+        double need = 0.0;
+        for (const SysDev &d : c->hs) {
+            any_skin = any_skin || d.skin_mode;
+            double rl2 = 0.0;
+            for (int t = 0; t < 16; ++t) rl2 = std::max(rl2, d.rl2[t]);
+            const double rho = d.n_part / (d.L[0] * d.L[1] * d.L[2]);
+            need = std::max(need, 2.0 * rho * 4.18879020478639 * rl2 * sqrt(rl2));
+        }
+        cap = any_skin ? std::min(1024, ((int)ceil(need) + 16 + 7) / 8 * 8) : 0;
+    }
+    if (cap == 0) return 0;
+    if (c->S_tot >= (1 << (31 - 7))) {
+        if (c->nl_request > 0) return fail(MFA_E_ARG, "neighbour list: too many slots for the list encoding");
+        return 0;
+    }
+    HIPCHK(hipMalloc((void **)&c->B.nl, (size_t)cap * c->S_tot * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&c->B.nl_n, (size_t)c->S_tot * sizeof(int)));
+    // lengths above any capacity until a bead's list is built: the pair
+    // kernels then scan the cells for it
+    HIPCHK(hipMemset(c->B.nl_n, 0x7f, (size_t)c->S_tot * sizeof(int)));
+    HIPCHK(hipDeviceSynchronize());
+    c->B.nl_cap = cap;
+    return 0;
+}
+
 static void enqueue_rebuild(mfa_ctx *c) {
     Timer t(c, K_REBUILD);
     const int g = std::min(GATED_GRID, std::max(1, c->n_blocks));
@@ -290,6 +336,12 @@ static void enqueue_rebuild(mfa_ctx *c) {
     else
         LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count);
+    if (c->B.nl_cap > 0) {
+        if (c->prec == MFA_PREC_MIXED)
+            LAUNCH((k_nlist<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange);
+        else
+            LAUNCH((k_nlist<false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange);
+    }
 }
 
 static void enqueue_vv1(mfa_ctx *c) {
@@ -459,6 +511,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
 #undef ALLOC
+    if ((rc = alloc_list(c))) { free_all(c); delete c; return rc; }
     // the zero fills of dalloc ran on the null stream: finish them before
     // anything is enqueued on the context's non-blocking stream
     if (hipDeviceSynchronize() != hipSuccess) { free_all(c); delete c; return fail(MFA_E_HIP, "allocation sync failed"); }
@@ -719,6 +772,7 @@ int mfa_hip_get_flags(mfa_ctx *c, mfa_flags *out) {
     out->nonfinite = f[0];
     out->out_of_box_z = f[1];
     out->fene_stretched = f[2];
+    out->list_overflow = f[4];
     out->rebuilds = rb;
     return 0;
 }
@@ -862,6 +916,27 @@ int mfa_hip_get_kernel_time(mfa_ctx *c, const char *kernel, double *total_ms, in
     for (int k = 0; k < K_NK; ++k)
         if (!strcmp(kernel, kNames[k])) { *total_ms = c->k_ms[k]; *launches = c->k_n[k]; return 0; }
     return fail(MFA_E_ARG, std::string("unknown kernel ") + kernel);
+}
+
+int mfa_hip_set_neighbor_list(mfa_ctx *c, int32_t capacity) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (capacity < -1 || capacity > 4096) return fail(MFA_E_ARG, "set_neighbor_list: capacity must be -1, 0 or 1..4096");
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->nl_request = capacity;
+    invalidate_graph(c);
+    if (int rc = alloc_list(c)) return rc;
+    // the new list is built by the next rebuild: schedule one for every channel
+    for (int s = 0; s < c->n_sys; ++s)
+        hipLaunchKernelGGL(k_mark_rebuild, dim3(1), dim3(1), 0, c->st, c->B, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_get_neighbor_list(mfa_ctx *c, int32_t *capacity) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (!capacity) return fail(MFA_E_ARG, "get_neighbor_list: null");
+    *capacity = c->B.nl_cap;
+    return 0;
 }
 
 int mfa_hip_set_virial(mfa_ctx *c, int32_t on) {

@@ -34,6 +34,7 @@ struct SysDev {
     double const_force[3], head_vel[2];
     double mass_type[4], inv_mass_type[4];
     double skin_half_2;
+    double rl2[16];               // Verlet-list radius^2 per type pair (range + skin), < 0: never listed
     uint64_t seed;
     int nc[3], n_cells_tot, cell_base;
     int slot_base, n_part, part_base;
@@ -62,8 +63,11 @@ struct Bufs {
     int *parity;
     int *rebuild;       // any channel rebuilds this step (gates the rebuild kernels)
     int *chan_rb;       // per channel: this channel's cells are rebuilt this step
-    int *flags;         // [0..3] anomaly counters, [6] offending index, [7] first guard code
+    int *flags;         // [0..3] anomaly counters, [4] list overflows, [6] offending index, [7] first guard code
     long long *t_idx;
+    int *nl;            // Verlet list, column-major: entry q of slot k at nl[q * n_slots + k]
+    int *nl_n;          // list length per slot (> nl_cap: overflowed, the cells are scanned)
+    int nl_cap;         // entries per slot; 0: no list, the pair kernels scan the cells
     int n_slots, n_beads, n_cells, n_heads;   // buffer extents, for the guards
 };
 

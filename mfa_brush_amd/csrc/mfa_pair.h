@@ -3,9 +3,20 @@
 // One thread per bead (sorted slot), full neighbour shell, no Newton-3
 // write to the partner: every unordered pair is evaluated by both of its
 // beads.  The two evaluations are exact negatives of each other (the minimum
-// image and every product below are odd in delta_r, and the DPD normal is
-// keyed on the unordered pair), so no atomics are needed and each bead's
-// force is a fixed-order register sum, hence deterministic.
+// image, the FP32 cell-frame difference below and every product are odd in
+// delta_r, and the DPD normal is keyed on the unordered pair), so no atomics
+// are needed and each bead's force is a fixed-order register sum, hence
+// deterministic, and the pair forces conserve momentum up to the rounding of
+// those sums.
+//
+// Partners come from a Verlet list built when the bead's channel rebuilds its
+// cells (k_nlist): every bead of the 27 (or 125) neighbour cells closer than
+// the pair type's interaction range plus the skin.  The skin test (every
+// bead within skin/2 of its position at that rebuild) keeps the list
+// complete until the next rebuild, so the interacting pairs are exactly the
+// ones a cell scan would find.  The list keeps the cell-scan order (dz, dy,
+// dx, slot), and a bead whose list overflowed scans the cells instead, in the
+// same order with the same arithmetic, so both routes give the same bits.
 //
 // Arithmetic follows fluid_fluid.f90:97-154 and dpd_forces.f90:45-115.
 #pragma once
@@ -18,6 +29,15 @@ struct PairAcc {
     double w[6];
     int np;
 };
+
+// list entry: partner slot << NL_CODE_BITS | cell-offset code, the code being
+// (dx+S) + (2S+1)((dy+S) + (2S+1)(dz+S)) for the partner's cell relative to
+// the bead's cell
+constexpr int NL_CODE_BITS = 7;
+__device__ __forceinline__ int nl_code(int dx, int dy, int dz, int S) {
+    const int w = 2 * S + 1;
+    return (dx + S) + w * ((dy + S) + w * (dz + S));
+}
 
 // One candidate j for bead i in FP64.  Returns nothing; accumulates into a.
 template <bool DPD, bool VIR>
@@ -108,10 +128,93 @@ __device__ __forceinline__ void load_tables(const SysDev &P, double *tab) {
     }
 }
 
+
 // ---------------------------------------------------------------------------
-// FP64 pair kernel.  Neighbour cells are visited as x-strips: for each
-// (dy, dz) the cells cx-S..cx+S of a row are contiguous slots unless the row
-// wraps, so one strip is one or two slot ranges.
+// Verlet-list build (after a rebuild, for the channels that rebuilt).  A
+// partner is listed when its distance is below (range + skin) per type pair
+// (rl2; < 0 for pairs that never interact).  The distance only has to select
+// a superset of the pairs that can interact before the next rebuild, so it
+// is taken from the cell-frame coordinates with the cell offset: FP32 on the
+// mixed path (16-byte partner loads, threshold widened by 2^-12), FP64
+// otherwise.
+// ---------------------------------------------------------------------------
+template <bool MIXED>
+__global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+                                              Bufs B, const int2 *__restrict__ crange) {
+    if (!*B.rebuild) return;
+    MFA_CUR(B);
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    if (!B.chan_rb[s]) return;
+    const SysDev &P = sys[s];
+    __shared__ double rl2d[16];
+    __shared__ float rl2f[16];
+    if (threadIdx.x < 16) {
+        rl2d[threadIdx.x] = P.rl2[threadIdx.x];
+        rl2f[threadIdx.x] = (float)(P.rl2[threadIdx.x] * (1.0 + 1.0 / 4096.0));
+    }
+    __syncthreads();
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k - P.slot_base >= P.n_part) return;
+    const double4 *__restrict__ pos = CUR(B.pos);
+    const float4 *__restrict__ posf = CUR(B.posf);
+    const int *__restrict__ cell_of = CUR(B.cell_of);
+    int c = cell_of[k] - P.cell_base;
+    if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
+    const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    const int S = P.S;
+    const size_t stride = (size_t)B.n_slots;
+    const int cap = B.nl_cap;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 pfi = make_float4(0.f, 0.f, 0.f, 0.f);
+    int ti;
+    if (MIXED) { pfi = posf[k]; ti = fwtype(pfi.w); }
+    else { pi = pos[k]; ti = wtype(pi.w); }
+    const int tr = (ti - 1) & 3;
+    int n = 0;
+    for (int dz = -S; dz <= S; ++dz) {
+        const int z2 = cz + dz;
+        if (z2 < 0 || z2 >= nz) continue;
+        for (int dy = -S; dy <= S; ++dy) {
+            int y2 = cy + dy;
+            double sy = 0.0;                 // periodic image of the partner row
+            if (y2 < 0) { y2 += ny; sy = -P.L[1]; } else if (y2 >= ny) { y2 -= ny; sy = P.L[1]; }
+            const int row = P.cell_base + nx * (y2 + ny * z2);
+            for (int dx = -S; dx <= S; ++dx) {
+                int x2 = cx + dx;
+                double sx = 0.0;
+                if (x2 < 0) { x2 += nx; sx = -P.L[0]; } else if (x2 >= nx) { x2 -= nx; sx = P.L[0]; }
+                int2 rg = crange[row + x2];
+                if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
+                const int code = nl_code(dx, dy, dz, S);
+                const float ox = (float)dx * (float)P.l_cell[0], oy = (float)dy * (float)P.l_cell[1],
+                            oz = (float)dz * (float)P.l_cell[2];
+                for (int m = rg.x; m < rg.y; ++m) {
+                    if (m == k) continue;
+                    bool take;
+                    if (MIXED) {
+                        const float4 pj = posf[m];
+                        const float ex = (pfi.x - pj.x) - ox, ey = (pfi.y - pj.y) - oy, ez = (pfi.z - pj.z) - oz;
+                        take = ex * ex + ey * ey + ez * ez < rl2f[tr + 4 * ((fwtype(pj.w) - 1) & 3)];
+                    } else {
+                        const double4 pj = pos[m];
+                        const double ex = pi.x - (pj.x + sx), ey = pi.y - (pj.y + sy), ez = pi.z - pj.z;
+                        take = ex * ex + ey * ey + ez * ez < rl2d[tr + 4 * ((wtype(pj.w) - 1) & 3)];
+                    }
+                    if (take) {
+                        if (n < cap) B.nl[(size_t)n * stride + k] = (m << NL_CODE_BITS) | code;
+                        ++n;
+                    }
+                }
+            }
+        }
+    }
+    B.nl_n[k] = n;
+    if (n > cap) atomicAdd(&B.flags[4], 1);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pair kernel.
 // ---------------------------------------------------------------------------
 template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
@@ -150,33 +253,40 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         double vxi = 0.0, vyi = 0.0, vzi = 0.0;
         if (DPD) { double4 v = vel[k]; vxi = v.x; vyi = v.y; vzi = v.z; }
-        int c = cell_of[k] - P.cell_base;
-        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
-        const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
-        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-        const int S = P.S;
-        for (int dz = -S; dz <= S; ++dz) {
-            const int z2 = cz + dz;
-            if (z2 < 0 || z2 >= nz) continue;
-            for (int dy = -S; dy <= S; ++dy) {
-                int y2 = cy + dy;
-                y2 += (y2 < 0) ? ny : 0;
-                y2 -= (y2 >= ny) ? ny : 0;
-                const int row = P.cell_base + nx * (y2 + ny * z2);
-                int xa = cx - S, xb = cx + S;
-                int seg_a0, seg_a1, seg_b0 = 0, seg_b1 = -1;
-                if (xa >= 0 && xb < nx) { seg_a0 = xa; seg_a1 = xb; }
-                else if (xa < 0) { seg_a0 = 0; seg_a1 = xb; seg_b0 = nx + xa; seg_b1 = nx - 1; }
-                else { seg_a0 = xa; seg_a1 = nx - 1; seg_b0 = 0; seg_b1 = xb - nx; }
-                for (int seg = 0; seg < 2; ++seg) {
-                    const int c0 = seg == 0 ? seg_a0 : seg_b0, c1 = seg == 0 ? seg_a1 : seg_b1;
-                    if (c1 < c0) continue;
-                    const int m0 = crange[row + c0].x, m1 = crange[row + c1].y;
-                    if (!guard(B, m0 >= 0 && m0 <= m1 && m1 <= B.n_slots, G_SLOT, k)) continue;
-                    for (int m = m0; m < m1; ++m) {
-                        if (m == k) continue;
-                        pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m,
-                                         r_2_min, (uint64_t)t_idx, a);
+        const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
+        if (nn >= 0 && nn <= B.nl_cap) {
+            const size_t stride = (size_t)B.n_slots;
+            for (int q = 0; q < nn; ++q) {
+                const int m = B.nl[(size_t)q * stride + k] >> NL_CODE_BITS;
+                if (!guard(B, in_range(m, B.n_slots), G_SLOT, k)) continue;
+                pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m, r_2_min,
+                                 (uint64_t)t_idx, a);
+            }
+        } else {
+            int c = cell_of[k] - P.cell_base;
+            if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
+            const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            const int S = P.S;
+            for (int dz = -S; dz <= S; ++dz) {
+                const int z2 = cz + dz;
+                if (z2 < 0 || z2 >= nz) continue;
+                for (int dy = -S; dy <= S; ++dy) {
+                    int y2 = cy + dy;
+                    y2 += (y2 < 0) ? ny : 0;
+                    y2 -= (y2 >= ny) ? ny : 0;
+                    const int row = P.cell_base + nx * (y2 + ny * z2);
+                    for (int dx = -S; dx <= S; ++dx) {
+                        int x2 = cx + dx;
+                        x2 += (x2 < 0) ? nx : 0;
+                        x2 -= (x2 >= nx) ? nx : 0;
+                        int2 rg = crange[row + x2];
+                        if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
+                        for (int m = rg.x; m < rg.y; ++m) {
+                            if (m == k) continue;
+                            pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m,
+                                             r_2_min, (uint64_t)t_idx, a);
+                        }
                     }
                 }
             }
@@ -199,11 +309,82 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
 
 // ---------------------------------------------------------------------------
 // Mixed-precision pair kernel: positions relative to the build-time cell
-// origin in FP32 (posf), FP32 pair arithmetic, per-cell FP32 partial sums
-// flushed into FP64 accumulators.  Pairs whose FP32 r^2 lies within a
-// relative band of 2^-14 around the cutoff are decided again in FP64 with the
-// reference arithmetic, so the interacting-pair set is the FP64 one.
+// origin in FP32 (posf), FP32 pair arithmetic, each pair force added into
+// FP64 accumulators.  Pairs whose FP32 r^2 lies within a relative band of
+// 2^-14 around the cutoff are decided again in FP64 with the reference
+// arithmetic, so the interacting-pair set is the FP64 one.
 // ---------------------------------------------------------------------------
+struct Pair32Ctx {
+    float r_2_min, rdpd2, inv_rdpd, sigf, gamf;
+    uint64_t t_idx;
+};
+
+// One candidate m for bead k.  The FP32 separation is (x_k - x_m) - o, with o
+// the partner cell's offset (d * l_cell, exact for |d| <= 2): a difference of
+// the two cell-frame coordinates first, so the partner's evaluation computes
+// exactly the negated value.
+template <bool DPD, bool VIR>
+__device__ __forceinline__ void pair32(const SysDev &P, const float *tf, const Pair32Ctx &X,
+                                       const double4 *__restrict__ pos, const float4 *__restrict__ velf,
+                                       int k, int ti, int oi, float4 pf, float4 vi, float ox, float oy, float oz,
+                                       float4 pj, int m, PairAcc &a) {
+    const float ddx = (pf.x - pj.x) - ox, ddy = (pf.y - pj.y) - oy, ddz = (pf.z - pj.z) - oz;
+    float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    const int tj = fwtype(pj.w);
+    const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
+    const bool maybe_dpd = DPD && P.dpd_cut_off && r2 < X.rdpd2 * 1.0001f;
+    if (!(r2 < tf[48 + tt]) && !maybe_dpd) return;
+    bool lj = r2 < tf[32 + tt];
+    if (!lj && r2 < tf[48 + tt]) {
+        // borderline: decide with the FP64 reference arithmetic
+        const double4 pa = pos[k], pb = pos[m];
+        double ex = mic(pa.x - pb.x, P.L[0], P.invL[0]);
+        double ey = mic(pa.y - pb.y, P.L[1], P.invL[1]);
+        double ez = pa.z - pb.z;
+        lj = r2_exact(ex, ey, ez) < P.range_2[tt];
+    }
+    float inv_sqrt = 0.f;
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f;
+    if (lj) {
+        a.np += 1;
+        if (P.force_switch_on && r2 < X.r_2_min) r2 = X.r_2_min;
+        const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
+        inv_sqrt = __builtin_amdgcn_rsqf(r2);
+        const float l_eps = tf[tt];
+        const float r_61 = tf[16 + tt] * inv_r_2;
+        const float r_6 = r_61 * r_61 * r_61;
+        const float r_12 = r_6 * r_6;
+        a.e += (double)(l_eps * ((r_12 - r_6) - tf[64 + tt]));
+        const float rd = l_eps * (6.0f * r_6 - 12.0f * r_12) * inv_r_2;
+        pfx = -rd * ddx; pfy = -rd * ddy; pfz = -rd * ddz;
+    }
+    if (DPD) {
+        bool do_dpd;
+        float wgt;
+        if (P.dpd_cut_off) {
+            do_dpd = ti != 1 && tj != 1 && r2 < X.rdpd2;
+            if (do_dpd && !lj) inv_sqrt = __builtin_amdgcn_rsqf(r2);
+            wgt = 1.0f - r2 * inv_sqrt * X.inv_rdpd;
+        } else {
+            do_dpd = lj;
+            wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
+        }
+        if (do_dpd) {
+            const float g = pair_gauss(P.seed, X.t_idx, oi - P.part_base, fworig(pj.w) - P.part_base);
+            const float rvx = ddx * inv_sqrt, rvy = ddy * inv_sqrt, rvz = ddz * inv_sqrt;
+            const float4 vj = velf[m];
+            const float rdv = (vi.x - vj.x) * rvx + (vi.y - vj.y) * rvy + (vi.z - vj.z) * rvz;
+            const float sc = X.sigf * wgt * g - X.gamf * (wgt * wgt) * rdv;
+            pfx += sc * rvx; pfy += sc * rvy; pfz += sc * rvz;
+        }
+    }
+    a.fx += (double)pfx; a.fy += (double)pfy; a.fz += (double)pfz;
+    if (VIR) {
+        a.w[0] += (double)(ddx * pfx); a.w[1] += (double)(ddy * pfy); a.w[2] += (double)(ddz * pfz);
+        a.w[3] += (double)(ddx * pfy); a.w[4] += (double)(ddx * pfz); a.w[5] += (double)(ddy * pfz);
+    }
+}
+
 template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
@@ -217,9 +398,12 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;
     __shared__ float tf[96];
+    __shared__ float4 off[128];         // cell offset (dx lx, dy ly, dz lz) per list code
     __shared__ double red[BLK];
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     const SysDev &P = sys[s];
+    const int S = P.S;
+    const float lx = (float)P.l_cell[0], ly = (float)P.l_cell[1], lz = (float)P.l_cell[2];
     {
         const int t = threadIdx.x;
         if (t < 16) {
@@ -230,132 +414,88 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
             tf[64 + t] = P.e_shift_f[t];
             tf[80 + t] = P.inv_rc_f[t];
         }
+        if (t < 128) {
+            const int w = 2 * S + 1;
+            const int dx = t % w - S, dy = (t / w) % w - S, dz = t / (w * w) - S;
+            off[t] = make_float4((float)dx * lx, (float)dy * ly, (float)dz * lz, 0.f);
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
     if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
     __syncthreads();
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
-    const long long t_idx = *t_dev;
-    float r_2_min = 0.0f;
-    double r_2_min_d = 0.0;
+    Pair32Ctx X;
+    X.t_idx = (uint64_t)*t_dev;
+    X.r_2_min = 0.0f;
     if (P.force_switch_on) {
-        double r_time = (double)t_idx * P.dt;
+        const double r_time = (double)X.t_idx * P.dt;
+        double r_2_min_d = 0.0;
         if (r_time < P.r_2_min_time) r_2_min_d = (1.0 - r_time / P.r_2_min_time) * P.r_2_min_init;
-        r_2_min = (float)r_2_min_d;
+        X.r_2_min = (float)r_2_min_d;
     }
-    const float sigf = (float)P.sig, gamf = (float)P.friction;
-    const float rdpd2 = (float)P.r_cut_dpd_2;
-    const float inv_rdpd = (float)(1.0 / sqrt(P.r_cut_dpd_2 > 0.0 ? P.r_cut_dpd_2 : 1.0));
-    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-    double w[6] = {0, 0, 0, 0, 0, 0};
-    int np = 0;
+    X.sigf = (float)P.sig; X.gamf = (float)P.friction;
+    X.rdpd2 = (float)P.r_cut_dpd_2;
+    X.inv_rdpd = (float)(1.0 / sqrt(P.r_cut_dpd_2 > 0.0 ? P.r_cut_dpd_2 : 1.0));
+    PairAcc a;
+    a.fx = a.fy = a.fz = a.e = 0.0;
+    for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
+    a.np = 0;
     if (active) {
         const float4 pf = posf[k];
         const int ti = fwtype(pf.w), oi = fworig(pf.w);
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
         if (DPD) vi = velf[k];
-        int c = cell_of[k] - P.cell_base;
-        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
-        const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
-        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-        const int S = P.S;
-        const float lx = (float)P.l_cell[0], ly = (float)P.l_cell[1], lz = (float)P.l_cell[2];
-        for (int dz = -S; dz <= S; ++dz) {
-            const int z2 = cz + dz;
-            if (z2 < 0 || z2 >= nz) continue;
-            const float qz = pf.z - (float)dz * lz;
-            for (int dy = -S; dy <= S; ++dy) {
-                int y2 = cy + dy;
-                y2 += (y2 < 0) ? ny : 0;
-                y2 -= (y2 >= ny) ? ny : 0;
-                const float qy = pf.y - (float)dy * ly;
-                const int row = P.cell_base + nx * (y2 + ny * z2);
-                for (int dx = -S; dx <= S; ++dx) {
-                    int x2 = cx + dx;
-                    x2 += (x2 < 0) ? nx : 0;
-                    x2 -= (x2 >= nx) ? nx : 0;
-                    const float qx = pf.x - (float)dx * lx;
-                    int2 rg = crange[row + x2];
-                    if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
-                    float cfx = 0.f, cfy = 0.f, cfz = 0.f, ce = 0.f;
-                    float cw[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    for (int m = rg.x; m < rg.y; ++m) {
-                        if (m == k) continue;
-                        const float4 pj = posf[m];
-                        const float ddx = qx - pj.x, ddy = qy - pj.y, ddz = qz - pj.z;
-                        float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        const int tj = fwtype(pj.w);
-                        const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
-                        const bool maybe_dpd = DPD && P.dpd_cut_off && r2 < rdpd2 * 1.0001f;
-                        if (!(r2 < tf[48 + tt]) && !maybe_dpd) continue;
-                        bool lj = r2 < tf[32 + tt];
-                        if (!lj && r2 < tf[48 + tt]) {
-                            // borderline: decide with the FP64 reference arithmetic
-                            const double4 a = pos[k], b = pos[m];
-                            double ex = mic(a.x - b.x, P.L[0], P.invL[0]);
-                            double ey = mic(a.y - b.y, P.L[1], P.invL[1]);
-                            double ez = a.z - b.z;
-                            lj = r2_exact(ex, ey, ez) < P.range_2[tt];
-                        }
-                        float inv_sqrt = 0.f;
-                        float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-                        if (lj) {
-                            np += 1;
-                            if (P.force_switch_on && r2 < r_2_min) r2 = r_2_min;
-                            const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
-                            inv_sqrt = __builtin_amdgcn_rsqf(r2);
-                            const float l_eps = tf[tt];
-                            const float r_61 = tf[16 + tt] * inv_r_2;
-                            const float r_6 = r_61 * r_61 * r_61;
-                            const float r_12 = r_6 * r_6;
-                            ce += l_eps * ((r_12 - r_6) - tf[64 + tt]);
-                            const float rd = l_eps * (6.0f * r_6 - 12.0f * r_12) * inv_r_2;
-                            pfx = -rd * ddx; pfy = -rd * ddy; pfz = -rd * ddz;
-                        }
-                        if (DPD) {
-                            bool do_dpd;
-                            float wgt;
-                            if (P.dpd_cut_off) {
-                                do_dpd = ti != 1 && tj != 1 && r2 < rdpd2;
-                                if (do_dpd && !lj) inv_sqrt = __builtin_amdgcn_rsqf(r2);
-                                wgt = 1.0f - r2 * inv_sqrt * inv_rdpd;
-                            } else {
-                                do_dpd = lj;
-                                wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
-                            }
-                            if (do_dpd) {
-                                const float g = pair_gauss(P.seed, (uint64_t)t_idx, oi - P.part_base, fworig(pj.w) - P.part_base);
-                                const float rvx = ddx * inv_sqrt, rvy = ddy * inv_sqrt, rvz = ddz * inv_sqrt;
-                                const float4 vj = velf[m];
-                                const float rdv = (vi.x - vj.x) * rvx + (vi.y - vj.y) * rvy + (vi.z - vj.z) * rvz;
-                                const float sc = sigf * wgt * g - gamf * (wgt * wgt) * rdv;
-                                pfx += sc * rvx; pfy += sc * rvy; pfz += sc * rvz;
-                            }
-                        }
-                        cfx += pfx; cfy += pfy; cfz += pfz;
-                        if (VIR) {
-                            cw[0] += ddx * pfx; cw[1] += ddy * pfy; cw[2] += ddz * pfz;
-                            cw[3] += ddx * pfy; cw[4] += ddx * pfz; cw[5] += ddy * pfz;
+        const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
+        if (nn >= 0 && nn <= B.nl_cap) {
+            const size_t stride = (size_t)B.n_slots;
+            for (int q = 0; q < nn; ++q) {
+                const int ent = B.nl[(size_t)q * stride + k];
+                const int m = ent >> NL_CODE_BITS;
+                if (!guard(B, in_range(m, B.n_slots), G_SLOT, k)) continue;
+                const float4 o = off[ent & ((1 << NL_CODE_BITS) - 1)];
+                pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, posf[m], m, a);
+            }
+        } else {
+            int c = cell_of[k] - P.cell_base;
+            if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
+            const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            for (int dz = -S; dz <= S; ++dz) {
+                const int z2 = cz + dz;
+                if (z2 < 0 || z2 >= nz) continue;
+                for (int dy = -S; dy <= S; ++dy) {
+                    int y2 = cy + dy;
+                    y2 += (y2 < 0) ? ny : 0;
+                    y2 -= (y2 >= ny) ? ny : 0;
+                    const int row = P.cell_base + nx * (y2 + ny * z2);
+                    for (int dx = -S; dx <= S; ++dx) {
+                        int x2 = cx + dx;
+                        x2 += (x2 < 0) ? nx : 0;
+                        x2 -= (x2 >= nx) ? nx : 0;
+                        int2 rg = crange[row + x2];
+                        if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
+                        const float4 o = off[nl_code(dx, dy, dz, S)];
+                        for (int m = rg.x; m < rg.y; ++m) {
+                            if (m == k) continue;
+                            pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, posf[m], m, a);
                         }
                     }
-                    fx += (double)cfx; fy += (double)cfy; fz += (double)cfz; e += (double)ce;
-                    if (VIR) for (int q = 0; q < 6; ++q) w[q] += (double)cw[q];
                 }
             }
         }
         double4 f = assign ? make_double4(0.0, 0.0, 0.0, 0.0) : frc[k];
-        f.x += fx; f.y += fy; f.z += fz;
+        f.x += a.fx; f.y += a.fy; f.z += a.fz;
         frc[k] = f;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
-    double es = block_sum(e, red);
-    double nps = block_sum((double)np, red);
+    double es = block_sum(a.e, red);
+    double nps = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }
     if (VIR) {
         for (int q = 0; q < 6; ++q) {
-            double wq = block_sum(w[q], red);
+            double wq = block_sum(a.w[q], red);
             if (threadIdx.x == 0) out[C_WXX + q] = wq;
         }
     }

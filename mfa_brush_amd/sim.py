@@ -229,6 +229,15 @@ class Simulation:
     def set_virial(self, on: bool = True):
         self._c(self.lib.mfa_hip_set_virial(self.h, int(on)), "set_virial")
 
+    def set_neighbor_list(self, capacity: int = -1):
+        """Verlet-list capacity per bead: -1 automatic, 0 cell scan only."""
+        self._c(self.lib.mfa_hip_set_neighbor_list(self.h, int(capacity)), "set_neighbor_list")
+
+    def neighbor_list(self) -> int:
+        n = C.c_int32()
+        self._c(self.lib.mfa_hip_get_neighbor_list(self.h, C.byref(n)), "get_neighbor_list")
+        return n.value
+
     def set_timing(self, on: bool = True):
         self._c(self.lib.mfa_hip_set_timing(self.h, int(on)), "set_timing")
 
