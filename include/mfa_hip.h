@@ -163,7 +163,7 @@ int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair
 
 /* Verlet-list capacity (entries per bead) of the pair kernels: -1 automatic
  * (default), 0 no list (the pair kernels scan the 27 or 125 neighbour cells
- * every step), > 0 explicit.  The list holds the partners within the
+ * every step), > 0 explicit (rounded up to a multiple of 8).  The list holds the partners within the
  * interaction range plus the skin; it is rebuilt with the cells, so the pair
  * set and the forces are the same bits with or without it.  No counterpart in
  * the reference (its BIN_TYPE 0/1 Verlet lists are host data structures,

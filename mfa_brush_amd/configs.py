@@ -181,3 +181,108 @@ def channel_scaled(n_side: int, n_mon: int = 32, lz: float = 60.0, density: floa
     p = channel_params((L, L, lz), n_mon, 2 * n_side * n_side, n_sol, **kw)
     a_type, r0, v = build_channel(p, a, n_side)
     return p, a_type, r0, v
+
+
+def _free_sites(L: float, lz: float, occupied: np.ndarray, spacing: float = 1.0, min_dist: float = 0.9,
+                z_margin: float = 0.8) -> np.ndarray:
+    """Sites of a cubic lattice (about `spacing` apart, fitted to the box)
+    farther than min_dist (minimum image in x, y) from every occupied bead."""
+    nxy = max(1, int(math.floor(L / spacing)))
+    sx = L / nxy
+    nz = max(1, int(math.floor((lz - 2 * z_margin) / spacing)) + 1)
+    sz = (lz - 2 * z_margin) / max(1, nz - 1)
+    g = (np.arange(nxy) + 0.5) * sx
+    xx, yy, zz = np.meshgrid(g, g, z_margin + sz * np.arange(nz), indexing="ij")
+    sites = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)
+    if len(occupied) == 0:
+        return sites
+    # cell grid over the occupied beads
+    nc = max(1, int(L // min_dist))
+    lc = L / nc
+    ncz = max(1, int(lz // min_dist) + 1)
+    key = lambda r: (np.floor(r[:, 0] / lc).astype(int) % nc, np.floor(r[:, 1] / lc).astype(int) % nc,
+                     np.clip(np.floor(r[:, 2] / min_dist).astype(int), 0, ncz - 1))
+    ox, oy, oz = key(occupied)
+    grid = {}
+    for i, c in enumerate(zip(ox, oy, oz)):
+        grid.setdefault(c, []).append(i)
+    sx_, sy_, sz_ = key(sites)
+    keep = np.ones(len(sites), bool)
+    for s in range(len(sites)):
+        cx, cy, cz = sx_[s], sy_[s], sz_[s]
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get(((cx + dx) % nc, (cy + dy) % nc, cz + dz), ()):
+                        d = sites[s] - occupied[j]
+                        d[:2] -= L * np.round(d[:2] / L)
+                        if d @ d < min_dist * min_dist:
+                            keep[s] = False
+                            break
+                    if not keep[s]:
+                        break
+                if not keep[s]:
+                    break
+    return sites[keep]
+
+
+def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density: float = 0.8,
+                    brush_z=None, solvent_weight=None, seed: int = 1, **kw):
+    """Two-brush channel of n_side^2 chains per wall at grafting spacing
+    a_graft (grafting density 1/a_graft^2), chains of n_mon beads, solvent to
+    the given total density.
+
+    Plain start: straight chains at bond 0.97 along z (heads at z = 1.2 and
+    lz - 1.2, the top lattice shifted by a_graft/2), solvent on the free sites
+    of a ~1 sigma cubic lattice, chosen uniformly.  Warm start: brush_z(m)
+    gives the height above its wall of monomer m (0 = head) and
+    solvent_weight(z) the relative probability of a solvent site at height z;
+    monomers that rise less than a bond per step step sideways (random
+    direction) so the bond stays 0.97."""
+    rng = np.random.default_rng(seed)
+    L = n_side * a_graft
+    n_chain = 2 * n_side * n_side
+    n_brush = n_chain * n_mon
+    n_sol = int(round(density * L * L * lz)) - n_brush
+    if n_sol < 0:
+        raise ValueError("brush alone exceeds the requested density")
+    p = channel_params((L, L, lz), n_mon, n_chain, n_sol, **kw)
+    g = (np.arange(n_side) + 0.5) * a_graft
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    base = np.stack([gx.ravel(), gy.ravel()], 1)
+    r0 = np.zeros((p.n_part, 3))
+    a_type = np.zeros(p.n_part, np.int32)
+    k = 0
+    for side in (0, 1):
+        heads = base if side == 0 else (base + a_graft / 2) % L
+        for c in range(len(heads)):
+            x, y = heads[c]
+            zprev = 1.2
+            for m in range(n_mon):
+                h = 1.2 + (BOND_Z * m if brush_z is None else float(brush_z(m)))
+                if m > 0:
+                    dz = min(abs(h - zprev), BOND_Z)
+                    lat = math.sqrt(max(BOND_Z * BOND_Z - dz * dz, 0.0))
+                    ang = rng.uniform(0, 2 * math.pi)
+                    x = (x + lat * math.cos(ang)) % L
+                    y = (y + lat * math.sin(ang)) % L
+                    h = zprev + dz
+                zprev = h
+                r0[k] = (x, y, h if side == 0 else lz - h)
+                a_type[k] = 1 if m == 0 else 2
+                k += 1
+    sites = _free_sites(L, lz, r0[:k])
+    if len(sites) < n_sol:
+        sites = _free_sites(L, lz, r0[:k], spacing=0.93)
+    if len(sites) < n_sol:
+        raise ValueError("not enough free sites for %d solvent beads (have %d)" % (n_sol, len(sites)))
+    w = None
+    if solvent_weight is not None:
+        w = np.maximum(np.asarray([solvent_weight(z) for z in sites[:, 2]], float), 1e-9)
+        w /= w.sum()
+    pick = rng.choice(len(sites), size=n_sol, replace=False, p=w)
+    r0[k:] = sites[np.sort(pick)]
+    a_type[k:] = 3
+    v = rng.normal(0.0, 1.0, size=(p.n_part, 3))
+    v[np.arange(n_chain) * n_mon] = 0.0
+    return p, a_type, r0, v

@@ -34,6 +34,13 @@ struct PairAcc {
 // (dx+S) + (2S+1)((dy+S) + (2S+1)(dz+S)) for the partner's cell relative to
 // the bead's cell
 constexpr int NL_CODE_BITS = 7;
+constexpr int NL_UNROLL = 4;      // partner loads issued together (one 16-byte list load)
+// Entry q of slot k: groups of four entries, slots interleaved per group, so
+// a wave reading group q/4 of its 64 slots reads one contiguous 1 KB span and
+// a slot's four consecutive entries share one 16-byte word.
+__device__ __forceinline__ size_t nl_index(int k, int q, size_t stride) {
+    return ((size_t)(q >> 2) * stride + (size_t)k) * 4 + (size_t)(q & 3);
+}
 __device__ __forceinline__ int nl_code(int dx, int dy, int dz, int S) {
     const int w = 2 * S + 1;
     return (dx + S) + w * ((dy + S) + w * (dz + S));
@@ -163,8 +170,8 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int S = P.S;
-    const size_t stride = (size_t)B.n_slots;
     const int cap = B.nl_cap;
+    const size_t stride = (size_t)B.n_slots;
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
     float4 pfi = make_float4(0.f, 0.f, 0.f, 0.f);
     int ti;
@@ -189,21 +196,31 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                 const int code = nl_code(dx, dy, dz, S);
                 const float ox = (float)dx * (float)P.l_cell[0], oy = (float)dy * (float)P.l_cell[1],
                             oz = (float)dz * (float)P.l_cell[2];
-                for (int m = rg.x; m < rg.y; ++m) {
-                    if (m == k) continue;
-                    bool take;
+                // four partner loads in flight per iteration (indices clamped
+                // into the range; the clamped duplicates are not used)
+                for (int m0 = rg.x; m0 < rg.y; m0 += NL_UNROLL) {
+                    bool take[NL_UNROLL];
                     if (MIXED) {
-                        const float4 pj = posf[m];
-                        const float ex = (pfi.x - pj.x) - ox, ey = (pfi.y - pj.y) - oy, ez = (pfi.z - pj.z) - oz;
-                        take = ex * ex + ey * ey + ez * ez < rl2f[tr + 4 * ((fwtype(pj.w) - 1) & 3)];
+                        float4 pj[NL_UNROLL];
+                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = posf[min(m0 + u, rg.y - 1)];
+                        for (int u = 0; u < NL_UNROLL; ++u) {
+                            const float ex = (pfi.x - pj[u].x) - ox, ey = (pfi.y - pj[u].y) - oy, ez = (pfi.z - pj[u].z) - oz;
+                            take[u] = ex * ex + ey * ey + ez * ez < rl2f[tr + 4 * ((fwtype(pj[u].w) - 1) & 3)];
+                        }
                     } else {
-                        const double4 pj = pos[m];
-                        const double ex = pi.x - (pj.x + sx), ey = pi.y - (pj.y + sy), ez = pi.z - pj.z;
-                        take = ex * ex + ey * ey + ez * ez < rl2d[tr + 4 * ((wtype(pj.w) - 1) & 3)];
+                        double4 pj[NL_UNROLL];
+                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[min(m0 + u, rg.y - 1)];
+                        for (int u = 0; u < NL_UNROLL; ++u) {
+                            const double ex = pi.x - (pj[u].x + sx), ey = pi.y - (pj[u].y + sy), ez = pi.z - pj[u].z;
+                            take[u] = ex * ex + ey * ey + ez * ez < rl2d[tr + 4 * ((wtype(pj[u].w) - 1) & 3)];
+                        }
                     }
-                    if (take) {
-                        if (n < cap) B.nl[(size_t)n * stride + k] = (m << NL_CODE_BITS) | code;
-                        ++n;
+                    for (int u = 0; u < NL_UNROLL; ++u) {
+                        const int m = m0 + u;
+                        if (m < rg.y && m != k && take[u]) {
+                            if (n < cap) B.nl[nl_index(k, n, stride)] = (m << NL_CODE_BITS) | code;
+                            ++n;
+                        }
                     }
                 }
             }
@@ -255,12 +272,20 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         if (DPD) { double4 v = vel[k]; vxi = v.x; vyi = v.y; vzi = v.z; }
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
-            const size_t stride = (size_t)B.n_slots;
-            for (int q = 0; q < nn; ++q) {
-                const int m = B.nl[(size_t)q * stride + k] >> NL_CODE_BITS;
-                if (!guard(B, in_range(m, B.n_slots), G_SLOT, k)) continue;
-                pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m, r_2_min,
-                                 (uint64_t)t_idx, a);
+            for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
+                int mm[NL_UNROLL];
+                double4 pj[NL_UNROLL];
+                const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
+                mm[0] = e4.x >> NL_CODE_BITS; mm[1] = e4.y >> NL_CODE_BITS;
+                mm[2] = e4.z >> NL_CODE_BITS; mm[3] = e4.w >> NL_CODE_BITS;
+#pragma unroll
+                for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[in_range(mm[u], B.n_slots) ? mm[u] : k];
+#pragma unroll
+                for (int u = 0; u < NL_UNROLL; ++u) {
+                    if (q0 + u < nn && guard(B, in_range(mm[u], B.n_slots), G_SLOT, k))
+                        pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj[u], vel, mm[u], r_2_min,
+                                         (uint64_t)t_idx, a);
+                }
             }
         } else {
             int c = cell_of[k] - P.cell_base;
@@ -449,13 +474,24 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
         if (DPD) vi = velf[k];
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
-            const size_t stride = (size_t)B.n_slots;
-            for (int q = 0; q < nn; ++q) {
-                const int ent = B.nl[(size_t)q * stride + k];
-                const int m = ent >> NL_CODE_BITS;
-                if (!guard(B, in_range(m, B.n_slots), G_SLOT, k)) continue;
-                const float4 o = off[ent & ((1 << NL_CODE_BITS) - 1)];
-                pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, posf[m], m, a);
+            for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
+                int ent[NL_UNROLL];
+                float4 pj[NL_UNROLL];
+                const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
+                ent[0] = e4.x; ent[1] = e4.y; ent[2] = e4.z; ent[3] = e4.w;
+#pragma unroll
+                for (int u = 0; u < NL_UNROLL; ++u) {
+                    const int m = ent[u] >> NL_CODE_BITS;
+                    pj[u] = posf[in_range(m, B.n_slots) ? m : k];
+                }
+#pragma unroll
+                for (int u = 0; u < NL_UNROLL; ++u) {
+                    const int m = ent[u] >> NL_CODE_BITS;
+                    if (q0 + u < nn && guard(B, in_range(m, B.n_slots), G_SLOT, k)) {
+                        const float4 o = off[ent[u] & ((1 << NL_CODE_BITS) - 1)];
+                        pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, pj[u], m, a);
+                    }
+                }
             }
         } else {
             int c = cell_of[k] - P.cell_base;

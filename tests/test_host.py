@@ -105,3 +105,15 @@ def test_fp32_divide_and_sqrt_from_fp64_are_correctly_rounded():
     for e in range(-24, 6):
         x = (np.arange(1 << 23, dtype=np.uint32) | np.uint32((e + 127) << 23)).view(np.float32)
         assert np.array_equal(np.sqrt(x.astype(np.float64)).astype(np.float32), np.sqrt(x)), e
+
+
+def test_mlp_header_binding_and_exports_agree():
+    """include/mfa_mlp.h, the ctypes table of mfa_brush_amd/surrogate.py and
+    the library's exports name the same entry points (no compute call)."""
+    from mfa_brush_amd import surrogate
+    txt = open(os.path.join(ROOT, "include", "mfa_mlp.h")).read()
+    declared = sorted(set(re.findall(r"\b(mfa_mlp_\w+)\s*\(", txt)))
+    assert sorted(surrogate.SIGNATURES) == declared
+    lib = C.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
