@@ -23,6 +23,7 @@ Workloads (sizes as stated, seeds fixed):
 from __future__ import annotations
 
 import math
+import warnings
 from dataclasses import replace
 from typing import Optional
 
@@ -187,6 +188,7 @@ def _free_sites(L: float, lz: float, occupied: np.ndarray, spacing: float = 1.0,
                 z_margin: float = 0.8) -> np.ndarray:
     """Sites of a cubic lattice (about `spacing` apart, fitted to the box)
     farther than min_dist (minimum image in x, y) from every occupied bead."""
+    from scipy.spatial import cKDTree
     nxy = max(1, int(math.floor(L / spacing)))
     sx = L / nxy
     nz = max(1, int(math.floor((lz - 2 * z_margin) / spacing)) + 1)
@@ -196,34 +198,92 @@ def _free_sites(L: float, lz: float, occupied: np.ndarray, spacing: float = 1.0,
     sites = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)
     if len(occupied) == 0:
         return sites
-    # cell grid over the occupied beads
-    nc = max(1, int(L // min_dist))
-    lc = L / nc
-    ncz = max(1, int(lz // min_dist) + 1)
-    key = lambda r: (np.floor(r[:, 0] / lc).astype(int) % nc, np.floor(r[:, 1] / lc).astype(int) % nc,
-                     np.clip(np.floor(r[:, 2] / min_dist).astype(int), 0, ncz - 1))
-    ox, oy, oz = key(occupied)
-    grid = {}
-    for i, c in enumerate(zip(ox, oy, oz)):
-        grid.setdefault(c, []).append(i)
-    sx_, sy_, sz_ = key(sites)
-    keep = np.ones(len(sites), bool)
-    for s in range(len(sites)):
-        cx, cy, cz = sx_[s], sy_[s], sz_[s]
+    box = [L, L, 4.0 * lz]          # periodic in x, y; z never wraps
+    occ = np.c_[occupied[:, 0] % L, occupied[:, 1] % L, occupied[:, 2] + lz]
+    tree = cKDTree(occ, boxsize=box)
+    d, _ = tree.query(np.c_[sites[:, 0] % L, sites[:, 1] % L, sites[:, 2] + lz], k=1)
+    return sites[d >= min_dist]
+
+
+class _Hash:
+    """Spatial hash of placed beads (periodic in x, y) for overlap checks."""
+
+    def __init__(self, L, cell=1.0):
+        self.L, self.n = L, max(1, int(L // cell))
+        self.c = L / self.n
+        self.d = {}
+
+    def _key(self, r):
+        return (int(r[0] // self.c) % self.n, int(r[1] // self.c) % self.n, int(math.floor(r[2] / self.c)))
+
+    def add(self, r):
+        self.d.setdefault(self._key(r), []).append(np.array(r, float))
+
+    def nearest(self, r, skip=None, reach=1.0):
+        """Distance to the nearest placed bead within `reach` (else reach)."""
+        kx, ky, kz = self._key(r)
+        best = reach * reach
         for dx in (-1, 0, 1):
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
-                    for j in grid.get(((cx + dx) % nc, (cy + dy) % nc, cz + dz), ()):
-                        d = sites[s] - occupied[j]
-                        d[:2] -= L * np.round(d[:2] / L)
-                        if d @ d < min_dist * min_dist:
-                            keep[s] = False
-                            break
-                    if not keep[s]:
-                        break
-                if not keep[s]:
+                    for q in self.d.get(((kx + dx) % self.n, (ky + dy) % self.n, kz + dz), ()):
+                        if skip is not None and q is skip:
+                            continue
+                        e = r - q
+                        e[:2] -= self.L * np.round(e[:2] / self.L)
+                        best = min(best, e @ e)
+        return math.sqrt(best)
+
+    def clear_of(self, r, dmin, skip=None):
+        kx, ky, kz = self._key(r)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in self.d.get(((kx + dx) % self.n, (ky + dy) % self.n, kz + dz), ()):
+                        if skip is not None and q is skip:
+                            continue
+                        e = r - q
+                        e[:2] -= self.L * np.round(e[:2] / self.L)
+                        if e @ e < dmin * dmin:
+                            return False
+        return True
+
+
+def _insert_solvent(L, lz, occupied, n_sol, rng, weight=None, dmin=0.85, z_margin=0.8, max_tries=400):
+    """Random sequential insertion of n_sol beads in [0,L)^2 x [z_margin,
+    lz - z_margin], each at least dmin (minimum image in x, y) from the
+    occupied beads and from the beads inserted before it.  Heights are
+    uniform, or drawn from weight(z) (inverse CDF on a 0.05 sigma grid)."""
+    from scipy.spatial import cKDTree
+    zs = np.arange(z_margin, lz - z_margin, 0.05)
+    cdf = None
+    if weight is not None:
+        w = np.maximum(np.asarray([weight(z) for z in zs], float), 1e-9)
+        cdf = np.cumsum(w)
+        cdf /= cdf[-1]
+    tree = cKDTree(np.c_[occupied[:, 0] % L, occupied[:, 1] % L, occupied[:, 2] + lz], boxsize=[L, L, 4 * lz]) \
+        if len(occupied) else None
+    hsh = _Hash(L)
+    out = np.zeros((n_sol, 3))
+    k, tries = 0, 0
+    while k < n_sol:
+        if tries > max_tries * max(n_sol, 1) // 4096 + 50:
+            raise ValueError("solvent insertion stalled at %d of %d beads" % (k, n_sol))
+        tries += 1
+        m = 4096
+        cand = np.c_[rng.uniform(0, L, m), rng.uniform(0, L, m),
+                     rng.uniform(z_margin, lz - z_margin, m) if cdf is None else np.interp(rng.uniform(0, 1, m), cdf, zs)]
+        if tree is not None:
+            d, _ = tree.query(np.c_[cand[:, 0], cand[:, 1], cand[:, 2] + lz], k=1)
+            cand = cand[d >= dmin]
+        for r in cand:
+            if hsh.clear_of(r, dmin):
+                hsh.add(r)
+                out[k] = r
+                k += 1
+                if k == n_sol:
                     break
-    return sites[keep]
+    return out
 
 
 def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density: float = 0.8,
@@ -233,12 +293,12 @@ def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density:
     the given total density.
 
     Plain start: straight chains at bond 0.97 along z (heads at z = 1.2 and
-    lz - 1.2, the top lattice shifted by a_graft/2), solvent on the free sites
-    of a ~1 sigma cubic lattice, chosen uniformly.  Warm start: brush_z(m)
-    gives the height above its wall of monomer m (0 = head) and
-    solvent_weight(z) the relative probability of a solvent site at height z;
-    monomers that rise less than a bond per step step sideways (random
-    direction) so the bond stays 0.97."""
+    lz - 1.2, the top lattice shifted by a_graft/2), solvent inserted at
+    uniform random positions at least 0.85 sigma from every bead.  Warm
+    start: brush_z(m) gives the height above its wall of monomer m (0 =
+    head) and solvent_weight(z) the relative density of solvent at height z;
+    monomers that rise less than a bond step sideways (random direction clear
+    of placed beads) so the bond stays 0.97."""
     rng = np.random.default_rng(seed)
     L = n_side * a_graft
     n_chain = 2 * n_side * n_side
@@ -253,35 +313,48 @@ def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density:
     r0 = np.zeros((p.n_part, 3))
     a_type = np.zeros(p.n_part, np.int32)
     k = 0
+    crowded = 0
+    hsh = _Hash(L) if brush_z is not None else None
     for side in (0, 1):
         heads = base if side == 0 else (base + a_graft / 2) % L
         for c in range(len(heads)):
             x, y = heads[c]
             zprev = 1.2
+            prev = None
             for m in range(n_mon):
                 h = 1.2 + (BOND_Z * m if brush_z is None else float(brush_z(m)))
-                if m > 0:
-                    dz = min(abs(h - zprev), BOND_Z)
-                    lat = math.sqrt(max(BOND_Z * BOND_Z - dz * dz, 0.0))
-                    ang = rng.uniform(0, 2 * math.pi)
-                    x = (x + lat * math.cos(ang)) % L
-                    y = (y + lat * math.sin(ang)) % L
-                    h = zprev + dz
+                if m > 0 and brush_z is not None:
+                    # rise towards the predicted height; the rest of the bond
+                    # goes sideways, in a direction clear of placed beads
+                    dz = min(max(h - zprev, 0.0), BOND_Z)
+                    best, best_d = None, -1.0
+                    for attempt in range(160):
+                        # past 80 tries the rise grows towards a straight bond
+                        dzt = dz if attempt < 80 else dz + (BOND_Z - dz) * (attempt - 79) / 80.0
+                        lat = math.sqrt(max(BOND_Z * BOND_Z - dzt * dzt, 0.0))
+                        ang = rng.uniform(0, 2 * math.pi)
+                        cand = np.array([(x + lat * math.cos(ang)) % L, (y + lat * math.sin(ang)) % L, zprev + dzt])
+                        zw = cand[2] if side == 0 else lz - cand[2]
+                        dn = hsh.nearest(np.array([cand[0], cand[1], zw]), skip=prev)
+                        if dn > best_d:
+                            best, best_d = cand, dn
+                        if dn >= 0.85:
+                            break
+                    if best_d < 0.85:
+                        crowded += 1    # the least crowded of the tries
+                    x, y, h = best
+                elif m > 0:
+                    h = zprev + BOND_Z
                 zprev = h
                 r0[k] = (x, y, h if side == 0 else lz - h)
+                if hsh is not None:
+                    hsh.add(r0[k])
+                    prev = hsh.d[hsh._key(r0[k])][-1]
                 a_type[k] = 1 if m == 0 else 2
                 k += 1
-    sites = _free_sites(L, lz, r0[:k])
-    if len(sites) < n_sol:
-        sites = _free_sites(L, lz, r0[:k], spacing=0.93)
-    if len(sites) < n_sol:
-        raise ValueError("not enough free sites for %d solvent beads (have %d)" % (n_sol, len(sites)))
-    w = None
-    if solvent_weight is not None:
-        w = np.maximum(np.asarray([solvent_weight(z) for z in sites[:, 2]], float), 1e-9)
-        w /= w.sum()
-    pick = rng.choice(len(sites), size=n_sol, replace=False, p=w)
-    r0[k:] = sites[np.sort(pick)]
+    if crowded:
+        warnings.warn("warm start: %d monomers closer than 0.85 sigma to another bead" % crowded)
+    r0[k:] = _insert_solvent(L, lz, r0[:k], n_sol, rng, solvent_weight)
     a_type[k:] = 3
     v = rng.normal(0.0, 1.0, size=(p.n_part, 3))
     v[np.arange(n_chain) * n_mon] = 0.0

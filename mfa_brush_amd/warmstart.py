@@ -1,0 +1,247 @@
+"""Surrogate warm start for brush channels.
+
+The surrogate (mfa_brush_amd.surrogate.Mlp, BF16 MFMA kernels) maps
+(grafting density, chain length, brush-solvent epsilon, z) to the brush and
+solvent number densities at height z.  It is trained on equilibrium profiles
+that the batched GPU simulator produces for a set of channels, and its
+prediction for a new channel seeds the initial configuration
+(configs.channel_general: monomer heights from the predicted brush density,
+solvent sites weighted by the predicted solvent density).  The plain lattice
+start (straight chains) remains the fallback, and plain MD remains the judge:
+time to equilibrium is measured on the simulated profiles of both starts.
+
+Time to equilibrium: the first block of `block` steps after which the
+block-averaged brush and solvent profiles (n_bins bins over [0, Lz], mirrored
+about the mid-plane) stay within `tol` (L1 distance relative to the L1 norm of
+the reference) for `consecutive` blocks; the reference is the mean of the
+second half of a long plain run.  No counterpart in the reference code (its
+starts come from gen_brush.f90 / conf_default.f90).
+"""
+from __future__ import annotations
+
+import json
+import math
+import sys
+import time
+from dataclasses import dataclass, asdict
+
+import numpy as np
+
+from . import configs
+from .sim import Simulation
+from .surrogate import Mlp
+
+LZ = 40.0
+N_BINS = 64
+WIDTHS = [4, 128, 128, 128, 128, 2]
+
+
+@dataclass
+class Spec:
+    sigma_g: float       # grafting density (chains per sigma^2 per wall)
+    n_mon: int           # beads per grafted chain (head included)
+    eps_bs: float        # brush-solvent WCA epsilon
+    side: float = 20.0   # box side in sigma (rounded to whole grafting cells)
+    seed: int = 1
+
+    def a_graft(self):
+        return 1.0 / math.sqrt(self.sigma_g)
+
+    def n_side(self):
+        return max(2, int(round(self.side / self.a_graft())))
+
+
+def _soft(built):
+    """Both starts use the reference's force switch-on (FORCE_SWITCH_ON):
+    pair distances below r_min are clamped to r_min, with r_min^2 ramping
+    from 0.64 to 0 over the first 1 tau (200 steps)."""
+    p, t, r0, v = built
+    return p.replace(r_2_min_init=0.64, r_2_min_time=1.0), t, r0, v
+
+
+def build(spec: Spec, mlp: Mlp | None = None, skin: float = 0.3):
+    """Plain start (mlp None) or surrogate warm start of a channel."""
+    kw = dict(eps_bs=spec.eps_bs, thermostat=True, rc_dpd=None, skin=skin, seed=spec.seed, force_switch_on=True)
+    if mlp is None:
+        return _soft(configs.channel_general(spec.n_side(), spec.a_graft(), spec.n_mon, LZ, **kw))
+    zc, rb, rs = predict(mlp, spec)
+    half = zc <= LZ / 2
+    z0 = 1.2
+    zb, wb = zc[half], np.maximum(rb[half], 0.0) * (zc[half] >= z0)
+    cdf = np.cumsum(wb)
+    if cdf[-1] <= 0:
+        return _soft(configs.channel_general(spec.n_side(), spec.a_graft(), spec.n_mon, LZ, **kw))
+    cdf = cdf / cdf[-1]
+    n = spec.n_mon
+
+    def brush_z(m):
+        if m == 0:
+            return 0.0
+        q = (m - 0.5) / (n - 1)
+        return float(np.interp(q, cdf, zb)) - z0
+
+    ws = np.maximum(rs, 1e-3)
+
+    def solvent_weight(z):
+        return float(np.interp(z, zc, ws))
+
+    return _soft(configs.channel_general(spec.n_side(), spec.a_graft(), spec.n_mon, LZ, brush_z=brush_z,
+                                         solvent_weight=solvent_weight, **kw))
+
+
+def _bins():
+    dz = LZ / N_BINS
+    return (np.arange(N_BINS) + 0.5) * dz, dz
+
+
+def densities(p, dens, n_obs):
+    """Number densities (brush, solvent) per bin, mirrored about the mid-plane."""
+    _, dz = _bins()
+    area = p.boundary[0] * p.boundary[1]
+    rho = dens / (max(n_obs, 1) * area * dz)
+    return 0.5 * (rho + rho[:, ::-1])
+
+
+def features(spec: Spec, z):
+    z = np.asarray(z, float)
+    return np.stack([np.full_like(z, (spec.sigma_g - 0.275) / 0.225), np.full_like(z, (spec.n_mon - 20.0) / 12.0),
+                     np.full_like(z, (spec.eps_bs - 1.0) / 0.5), 2.0 * z / LZ - 1.0], 1).astype(np.float32)
+
+
+def predict(mlp: Mlp, spec: Spec):
+    zc, _ = _bins()
+    y = mlp.forward(features(spec, zc))
+    return zc, y[:, 0], y[:, 1]
+
+
+def equilibrium_profiles(specs, equil_steps: int, avg_steps: int, every: int = 100, skin: float = 0.3):
+    """Run all channels in one batched context; profiles averaged over the
+    last avg_steps steps.  Returns a list of (brush, solvent) densities."""
+    built = []
+    for i, s in enumerate(specs):
+        built.append(build(s, None, skin))
+        print("  built training channel %d/%d (%d beads)" % (i + 1, len(specs), built[-1][0].n_part), flush=True)
+    with Simulation([b[0] for b in built]) as sim:
+        for i, (p, t, r0, v) in enumerate(built):
+            sim.set_state(r0, v, t, isys=i)
+        done = 0
+        while done < equil_steps:
+            n = min(5000, equil_steps - done)
+            sim.md_steps(n, graph=True)
+            done += n
+            print("  equilibrate %d/%d" % (done, equil_steps), flush=True)
+        sim.profiles_reset(N_BINS)
+        for q in range(avg_steps // every):
+            sim.md_steps(every, graph=True)
+            sim.profiles_accumulate()
+        out = []
+        for i, (p, *_r) in enumerate(built):
+            n_obs, dens, _, _ = sim.profiles(i)
+            out.append(densities(p, dens, n_obs))
+        fl = sim.flags()
+    return out, fl, int(sum(b[0].n_part for b in built))
+
+
+def training_specs(n: int, seed: int = 11):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        specs.append(Spec(sigma_g=float(rng.uniform(0.05, 0.5)), n_mon=int(rng.choice([8, 16, 32])),
+                          eps_bs=float(rng.uniform(0.5, 1.5)), seed=100 + i))
+    return specs
+
+
+def train(specs, profiles, steps: int = 3000, seed: int = 5):
+    zc, _ = _bins()
+    X = np.concatenate([features(s, zc) for s in specs])
+    T = np.concatenate([np.stack([pr[0], pr[1]], 1) for pr in profiles]).astype(np.float32)
+    mlp = Mlp(WIDTHS, seed=seed)
+    hist = mlp.train(X, T, steps=steps, batch=512, lr=2e-3, seed=seed)
+    return mlp, hist
+
+
+def block_profiles(spec: Spec, mlp, n_blocks: int, block: int = 1000, every: int = 50, skin: float = 0.3):
+    """Block-averaged (brush, solvent) densities of one channel run."""
+    p, t, r0, v = build(spec, mlp, skin)
+    print("  %s start built: %d beads" % ("warm" if mlp is not None else "plain", p.n_part), flush=True)
+    out = []
+    t0 = time.perf_counter()
+    with Simulation(p) as sim:
+        sim.set_state(r0, v, t)
+        for b in range(n_blocks):
+            sim.profiles_reset(N_BINS)
+            for q in range(block // every):
+                sim.md_steps(every, graph=True)
+                sim.profiles_accumulate()
+            n_obs, dens, _, _ = sim.profiles()
+            out.append(densities(p, dens, n_obs))
+            if (b + 1) % 20 == 0:
+                print("  block %d/%d" % (b + 1, n_blocks), flush=True)
+        sim.synchronize()
+        fl = sim.flags()
+    return np.array(out), p.n_part, time.perf_counter() - t0, fl
+
+
+def l1_error(prof, ref):
+    return float(np.abs(prof - ref).sum() / np.abs(ref).sum())
+
+
+def time_to_equilibrium(blocks, ref, tol: float = 0.02, consecutive: int = 5):
+    """Index (1-based, in blocks) of the first block after which `consecutive`
+    blocks stay within tol of ref; None if never."""
+    err = [l1_error(b, ref) for b in blocks]
+    for i in range(len(err) - consecutive + 1):
+        if all(e <= tol for e in err[i:i + consecutive]):
+            return i + 1, err
+    return None, err
+
+
+def main(argv=None):
+    """Train on batched runs, then compare plain and warm starts on held-out
+    channels.  Arguments: report path, training channels, equilibration and
+    averaging steps of the training runs, blocks of 1000 steps per held-out
+    run, held-out box side (sigma).  Writes a JSON report."""
+    argv = sys.argv[1:] if argv is None else argv
+    out_path = argv[0] if argv else "warmstart_report.json"
+    n_train = int(argv[1]) if len(argv) > 1 else 48
+    equil, avg = (int(argv[2]), int(argv[3])) if len(argv) > 3 else (40000, 20000)
+    n_blocks = int(argv[4]) if len(argv) > 4 else 100
+    side = float(argv[5]) if len(argv) > 5 else 56.0
+    rep = {"n_train": n_train, "equil_steps": equil, "avg_steps": avg, "n_bins": N_BINS, "lz": LZ}
+    t0 = time.perf_counter()
+    specs = training_specs(n_train)
+    profs, fl, nb = equilibrium_profiles(specs, equil, avg)
+    rep["train_sim_seconds"] = time.perf_counter() - t0
+    rep["train_beads"] = nb
+    rep["train_flags"] = fl
+    print("training profiles: %.1f s" % rep["train_sim_seconds"], flush=True)
+    t0 = time.perf_counter()
+    mlp, hist = train(specs, profs)
+    rep["train_mlp_seconds"] = time.perf_counter() - t0
+    rep["loss_first_last"] = [float(hist[:20].mean()), float(hist[-20:].mean())]
+    held = [Spec(0.30, 32, 1.0, side=side, seed=901), Spec(0.12, 16, 0.8, side=side, seed=902),
+            Spec(0.45, 32, 1.3, side=side, seed=903)]
+    rep["held_out"] = []
+    for s in held:
+        plain, n_part, secs_p, flp = block_profiles(s, None, n_blocks)
+        warm, _, secs_w, flw = block_profiles(s, mlp, n_blocks)
+        ref = plain[n_blocks // 2:].mean(0)
+        tp, ep = time_to_equilibrium(plain, ref)
+        tw, ew = time_to_equilibrium(warm, ref)
+        zc, rb, rs = predict(mlp, s)
+        pred_err = l1_error(np.stack([rb, rs]), ref)
+        warm_final = l1_error(warm[n_blocks // 2:].mean(0), ref)
+        r = {"spec": asdict(s), "n_part": n_part, "blocks": n_blocks, "block_steps": 1000,
+             "tte_plain_blocks": tp, "tte_warm_blocks": tw, "speedup": (tp / tw) if (tp and tw) else None,
+             "surrogate_l1_vs_reference": pred_err, "warm_final_profile_l1": warm_final,
+             "err_plain": ep, "err_warm": ew, "seconds": [secs_p, secs_w],
+             "flags": [flp, flw]}
+        print(json.dumps(r), flush=True)
+        rep["held_out"].append(r)
+    with open(out_path, "w") as f:
+        json.dump(rep, f, indent=1)
+    print("report:", out_path, flush=True)
+
+
+if __name__ == "__main__":
+    main()
