@@ -28,15 +28,17 @@ PEAK_FP64 = 78.6e12          # MI355X vector FP64
 HBM_PEAK = 8.0e12
 
 
-def run_gpu(precision, steps, warmup, p, t, r0, v, nlist=-1):
+def run_gpu(precision, steps, warmup, ps, states, nlist=-1):
     """Warm-up steps (graph replay), then the timed region: `steps` steps with
     HIP events around every kernel group on the library's stream (so the
     per-kernel average durations come from the same steps as the wall time),
-    then the same number of steps as graph replays without events."""
+    then the same number of steps as graph replays without events.  ps and
+    states list the channels of one (possibly batched) context."""
     from mfa_brush_amd.sim import Simulation
-    s = Simulation(p, precision=precision)
+    s = Simulation(ps if len(ps) > 1 else ps[0], precision=precision)
     s.set_neighbor_list(nlist)
-    s.set_state(r0, v, t)
+    for i, (t, r0, v) in enumerate(states):
+        s.set_state(r0, v, t, isys=i)
     s.md_steps(warmup, graph=True)
     s.synchronize()
     s.set_timing(True)
@@ -50,14 +52,62 @@ def run_gpu(precision, steps, warmup, p, t, r0, v, nlist=-1):
     s.md_steps(steps, graph=True)
     s.synchronize()
     dt_graph = time.perf_counter() - t0
-    npairs = s.pair_count()
+    npairs = sum(s.pair_count(i) for i in range(len(ps)))
     fl = s.flags()
     cap = s.neighbor_list()
-    e = s.energies()
     s.close()
     pair_ms, pair_n = tot["pair"]
     return dict(seconds=dt, seconds_graph=dt_graph, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl,
-                energies=e, list_cap=cap, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+                list_cap=cap, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+
+
+def _c4_replica(i):
+    """Replica i of the batched workload: size log-uniform in [2k, 40k]
+    beads, grafting density U[0.05, 0.5], N in {8, 16, 32, 64}, brush-solvent
+    epsilon U[0.5, 1.5], DPD rc 1; per-replica seeds.  Lz is 30, raised
+    where straight chains from both walls would not fit (N 32 and 64) or the
+    two brushes alone would exceed 5/8 of the beads at density 0.8."""
+    import math
+    import numpy as np
+    from mfa_brush_amd import configs
+    rng = np.random.default_rng(1000 + i)
+    n_target = math.exp(rng.uniform(math.log(2000), math.log(40000)))
+    sg = rng.uniform(0.05, 0.5)
+    n_mon = int(rng.choice([8, 16, 32, 64]))
+    eps = rng.uniform(0.5, 1.5)
+    lz = max(30.0, 2.0 * sg * n_mon / 0.5, 2.0 * (1.2 + 0.97 * (n_mon - 1)) + 2.0)
+    a = 1.0 / math.sqrt(sg)
+    side = max(9.0, math.sqrt(n_target / (0.8 * lz)))
+    n_side = max(2, int(round(side / a)))
+    while n_side * a < 9.0:
+        n_side += 1
+    p, t, r0, v = configs.channel_general(n_side, a, n_mon, lz, eps_bs=eps, thermostat=True,
+                                          rc_dpd=1.0, seed=0x5EED0001 + i, force_switch_on=True)
+    # force switch-on ramp over the first 1 tau for crowded starts
+    return p.replace(r_2_min_init=0.64, r_2_min_time=1.0), t, r0, v
+
+
+def build_workload(name, skin, n_rep=256):
+    """(params list, states list, description) of a bench workload."""
+    from mfa_brush_amd import configs
+    if name == "c2":
+        p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=skin)
+        return [p], [(t, r0, v)], ("brush channel 998,784 beads, box 144.25x144.25x60, 2x10404 chains N=32, "
+                                   "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005")
+    if name == "c3":
+        p, t, r0, v = configs.channel_melt(thermostat=True, rc_dpd=1.0, skin=skin)
+        return [p], [(t, r0, v)], ("melt channel 1,200,000 beads, 18,750 chains N=64, box 216.93x216.93x30, "
+                                   "FENE + harmonic bending k_b 5, Poiseuille g 0.01 x, DPD rc 1, dt 0.005")
+    if name == "c4":
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            reps = list(ex.map(_c4_replica, range(n_rep)))
+        ps = [r[0].replace(skin=skin) for r in reps]
+        return ps, [(r[1], r[2], r[3]) for r in reps], (
+            "%d brush-channel replicas, 2k-40k beads log-uniform, grafting density U[0.05,0.5], N in "
+            "{8,16,32,64}, eps_bs U[0.5,1.5], Lz 30 (more where straight chains need it), DPD rc 1, one batched "
+            "context" % n_rep)
+    raise ValueError(name)
 
 
 def traffic_from_profile():
@@ -86,19 +136,24 @@ def calib_copy(mib):
     print("calib copy: %d MiB x 3" % mib, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, t, r0, v, n_steps):
+def cpu_baseline(ps, states, n_steps, max_channels=16):
     """The oracle's OpenMP build (same algorithm as the reference: linked
     cells, half shell, full-array force reduction, serial integrator) on
-    this host's cores, on a bounded number of steps of the same workload."""
+    this host's cores, on a bounded sample: n_steps steps of the first
+    max_channels channels of the workload."""
     import numpy as np
     from oracle.oracle import Oracle
-    o = Oracle(p, omp=True)
-    threads = o.lib.orc_num_threads()
-    t0 = time.perf_counter()
-    o.md_steps(r0, v, np.zeros_like(r0), t, 0, n_steps, noise=1)
-    dt = time.perf_counter() - t0
-    return dict(value=p.n_part * n_steps / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample="%d steps of the full %d-bead workload (%.1f s)" % (n_steps, p.n_part, dt))
+    beads, dt, threads = 0, 0.0, 1
+    for p, (t, r0, v) in list(zip(ps, states))[:max_channels]:
+        o = Oracle(p, omp=True)
+        threads = o.lib.orc_num_threads()
+        t0 = time.perf_counter()
+        o.md_steps(r0, v, np.zeros_like(r0), t, 0, n_steps, noise=1)
+        dt += time.perf_counter() - t0
+        beads += p.n_part
+    return dict(value=beads * n_steps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample="%d steps of %d channel(s) of the workload, %d beads (%.1f s)" %
+                       (n_steps, min(len(ps), max_channels), beads, dt))
 
 
 def main():
@@ -112,6 +167,10 @@ def main():
     ap.add_argument("--no-list", action="store_true", help="pair kernels scan the cells every step")
     ap.add_argument("--skin", type=float, default=0.4,
                     help="Verlet skin (cells and lists); the pair set does not depend on it")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
+                    help="c2: 1M-bead brush channel (the metric); c3: 1.2M-bead melt with bending and "
+                         "Poiseuille flow; c4: 256 batched replicas")
+    ap.add_argument("--autotune", action="store_true", help="pick the skin by timing short runs first")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -122,17 +181,23 @@ def main():
 
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib)
-    from mfa_brush_amd import configs
-    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0, skin=args.skin)
-    n = p.n_part
-    print("workload: %d beads, box %s" % (n, p.boundary), file=sys.stderr, flush=True)
+    ps, states, desc = build_workload(args.workload, args.skin)
+    n = sum(p.n_part for p in ps)
+    print("workload %s: %d channel(s), %d beads" % (args.workload, len(ps), n), file=sys.stderr, flush=True)
+    tuned = None
+    if args.autotune:
+        from mfa_brush_amd import autotune
+        tuned = autotune.tune(ps, states, "mixed")
+        print("autotune:", json.dumps(tuned), file=sys.stderr, flush=True)
+        ps = [p.replace(skin=tuned["skin"]) for p in ps]
+        args.skin = tuned["skin"]
 
     nlist = 0 if args.no_list else -1
-    mixed = run_gpu("mixed", args.steps, args.warmup, p, t, r0, v, nlist)
+    mixed = run_gpu("mixed", args.steps, args.warmup, ps, states, nlist)
     print("mixed: %.3f s for %d steps" % (mixed["seconds"], args.steps), file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_gpu("fp64", args.steps, args.warmup, p, t, r0, v, nlist)
+        fp64 = run_gpu("fp64", args.steps, args.warmup, ps, states, nlist)
         print("fp64: %.3f s for %d steps" % (fp64["seconds"], args.steps), file=sys.stderr, flush=True)
 
     value = n * args.steps / mixed["seconds"]
@@ -151,9 +216,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32-pair/fp64-integrate",
         "data": "synthetic (seeded column-lattice start, Maxwell-Boltzmann velocities)",
-        "config": {"workload": "brush channel 998,784 beads, box 144.25x144.25x60, 2x10404 chains N=32, "
-                               "332,928 solvent, LJ rc 2.5 + DPD rc 1, dt 0.005",
-                   "skin": args.skin,
+        "config": {"workload": desc, "skin": args.skin, "channels": len(ps),
                    "n_beads": n, "interacting_pairs": mixed["npairs"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile(),
@@ -174,7 +237,9 @@ def main():
                        "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in fp64["kernels"].items()}}
         out["mixed_over_fp64"] = value / v64
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, t, r0, v, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(ps, states, args.cpu_steps)
+    if tuned is not None:
+        out["autotune"] = tuned
     print(json.dumps(out), flush=True)
 
 

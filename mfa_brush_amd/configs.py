@@ -359,3 +359,43 @@ def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density:
     v = rng.normal(0.0, 1.0, size=(p.n_part, 3))
     v[np.arange(n_chain) * n_mon] = 0.0
     return p, a_type, r0, v
+
+
+def channel_melt(n_cols: int = 200, n_z: int = 30, n_mon: int = 64, lz: float = 30.0, density: float = 0.85,
+                 k_bend: float = 5.0, g: float = 0.01, seed: int = 3, **kw):
+    """Melt in a channel (no brush): n_cols^2 columns of n_z sites fill the
+    box at `density`; chains of n_mon beads follow a serpentine path through
+    the columns (up one column, down the next, rows of columns in
+    alternating x direction), so every bond is either the in-column site
+    spacing or the column spacing.  FENE + harmonic-angle bending
+    (bending_melt, k_bend) and a Poiseuille body force g along x on every
+    bead.  The default is the 1.2M-bead melt (18,750 chains of 64,
+    L = 216.93, Lz = 30)."""
+    n_sites = n_cols * n_cols * n_z
+    if n_sites % n_mon:
+        raise ValueError("columns x sites must be a multiple of the chain length")
+    L = math.sqrt(n_sites / (density * lz))
+    ac = L / n_cols
+    zs = np.linspace(0.7, lz - 0.7, n_z)
+    if min(ac, zs[1] - zs[0]) < 0.9 or max(ac, zs[1] - zs[0]) > 1.4:
+        raise ValueError("site spacing out of range (%.3f, %.3f)" % (ac, zs[1] - zs[0]))
+    rng = np.random.default_rng(seed)
+    p = channel_params((L, L, lz), 0, 0, n_sites // n_mon, bending=False, k_bend=k_bend,
+                       const_force=(g, 0.0, 0.0), **kw)
+    p = p.replace(n_mon_d=n_mon, n_chain_d=n_sites // n_mon,
+                  flags=Flags(**{**p.flags.__dict__, "bending_melt": k_bend > 0.0, "poiseuille": g != 0.0}))
+    r0 = np.zeros((n_sites, 3))
+    k = 0
+    col = 0
+    for iy in range(n_cols):
+        xs = range(n_cols) if iy % 2 == 0 else range(n_cols - 1, -1, -1)
+        for ix in xs:
+            zz = zs if col % 2 == 0 else zs[::-1]
+            r0[k:k + n_z, 0] = (ix + 0.5) * ac
+            r0[k:k + n_z, 1] = (iy + 0.5) * ac
+            r0[k:k + n_z, 2] = zz
+            k += n_z
+            col += 1
+    a_type = np.full(n_sites, 3, np.int32)
+    v = rng.normal(0.0, 1.0, size=(n_sites, 3))
+    return p, a_type, r0, v

@@ -326,8 +326,8 @@ static void enqueue_rebuild(mfa_ctx *c) {
     LAUNCH(k_bin, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank, c->S_tot);
     LAUNCH(k_scan1, nsb, c->st, c->B.rebuild, c->d_cell_count, c->C_tot, c->d_pre, c->d_bsum);
     LAUNCH(k_scan2, 1, c->st, c->B.rebuild, c->d_bsum, nsb);
-    LAUNCH(k_scan3, gc, c->st, c->d_sys, c->B.rebuild, c->d_cell_count, c->d_pre, c->d_bsum, c->d_cell_sys,
-           c->d_crange, c->C_tot);
+    LAUNCH(k_scan3, gc, c->st, c->d_sys, c->B.rebuild, (const int *)c->B.chan_rb, c->d_cell_count, c->d_pre,
+           c->d_bsum, c->d_cell_sys, c->d_crange, c->C_tot);
     LAUNCH(k_scatter, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_bin_cell, c->d_bin_rank, c->d_crange,
            c->d_perm_tmp, c->S_tot);
     LAUNCH(k_rank, g, c->st, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
@@ -336,7 +336,7 @@ static void enqueue_rebuild(mfa_ctx *c) {
         LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     else
         LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
-    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count);
+    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count, c->n_sys);
     if (c->B.nl_cap > 0) {
         if (c->prec == MFA_PREC_MIXED)
             LAUNCH((k_nlist<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange);
@@ -469,7 +469,6 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
         d.head_base = (int)heads;
         d.block_base = (int)(slots / BLK);
         d.n_blocks = (d.n_part + BLK - 1) / BLK;
-        d.shift = (int)(slots - parts);
         slots += (long long)d.n_blocks * BLK;
         parts += d.n_part;
         cells += d.n_cells_tot;
@@ -499,7 +498,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->B.pos, 2 * (size_t)c->S_tot); ALLOC(c->B.vel, 2 * (size_t)c->S_tot); ALLOC(c->B.acc, 2 * (size_t)c->S_tot);
     ALLOC(c->B.posb, 2 * (size_t)c->S_tot); ALLOC(c->B.cell_of, 2 * (size_t)c->S_tot);
     if (precision == MFA_PREC_MIXED) { ALLOC(c->B.posf, 2 * (size_t)c->S_tot); ALLOC(c->B.velf, 2 * (size_t)c->S_tot); }
-    ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, 1); ALLOC(c->B.rebuild, 1);
+    ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, n_sys); ALLOC(c->B.rebuild, 1);
     ALLOC(c->B.chan_rb, n_sys);
     ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1);
     ALLOC(c->d_sys, n_sys); ALLOC(c->d_blk_sys, c->n_blocks); ALLOC(c->d_cell_sys, c->C_tot);
@@ -829,7 +828,7 @@ int mfa_hip_get_cells(mfa_ctx *c, int32_t isys, int32_t *n_cells, int32_t *cell_
     std::vector<int2> rg(d.n_cells_tot);
     if (int rc = to_host(c, rg.data(), c->d_crange + d.cell_base, d.n_cells_tot * sizeof(int2), "cell ranges")) return rc;
     int par = 0;
-    if (int rc = to_host(c, &par, c->B.parity, sizeof(int), "parity")) return rc;
+    if (int rc = to_host(c, &par, c->B.parity + isys, sizeof(int), "parity")) return rc;
     std::vector<double4> pos(d.n_part);
     const double4 *cur = c->B.pos + (size_t)(par & 1) * (size_t)c->B.n_slots;
     if (int rc = to_host(c, pos.data(), cur + d.slot_base, d.n_part * sizeof(double4), "positions")) return rc;

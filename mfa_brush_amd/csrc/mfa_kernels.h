@@ -42,14 +42,14 @@ struct SysDev {
     int thermostat, dpd_cut_off, force_switch_on, bending, bending_melt, poiseuille;
     int S, skin_mode;
     int block_base, n_blocks;
-    int shift;                    // slot_base - (beads of all earlier channels)
 };
 
-// Double-buffered bead arrays: a rebuild permutes into the other half and a
-// one-thread kernel flips `parity`, so no copy-back pass is needed and the
-// host never has to know whether the device decided to rebuild.  Each array
-// is one allocation of 2 * n_slots entries; the current half starts at
-// parity * n_slots.  The halves are reached by offset arithmetic on a single
+// Double-buffered bead arrays: a rebuild permutes a channel into the other
+// half and a one-thread kernel flips that channel's `parity`, so no copy-back
+// pass is needed, channels that do not rebuild are not touched, and the host
+// never has to know whether the device decided to rebuild.  Each array is one
+// allocation of 2 * n_slots entries; channel s's current half starts at
+// parity[s] * n_slots.  The halves are reached by offset arithmetic on a single
 // pointer, never by choosing between pointer fields of this struct at run
 // time: the AMDGPU backend of this toolchain left the pointer register
 // undefined on one branch of a run-time choice between pointers indexed
@@ -60,7 +60,7 @@ struct Bufs {
     int *cell_of;
     double4 *frc;
     int *where;
-    int *parity;
+    int *parity;        // per channel: which half holds the current layout
     int *rebuild;       // any channel rebuilds this step (gates the rebuild kernels)
     int *chan_rb;       // per channel: this channel's cells are rebuilt this step
     int *flags;         // [0..3] anomaly counters, [4] list overflows, [6] offending index, [7] first guard code
@@ -71,8 +71,8 @@ struct Bufs {
     int n_slots, n_beads, n_cells, n_heads;   // buffer extents, for the guards
 };
 
-#define MFA_CUR(B)                                                        \
-    const int par = *(B).parity & 1;                                      \
+#define MFA_CUR(B, s)                                                     \
+    const int par = (B).parity[s] & 1;                                    \
     const size_t cur_off = (size_t)par * (size_t)(B).n_slots;             \
     const size_t nxt_off = (size_t)(par ^ 1) * (size_t)(B).n_slots;       \
     (void)cur_off; (void)nxt_off

@@ -149,9 +149,9 @@ template <bool MIXED>
 __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                               Bufs B, const int2 *__restrict__ crange) {
     if (!*B.rebuild) return;
-    MFA_CUR(B);
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     if (!B.chan_rb[s]) return;
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     __shared__ double rl2d[16];
     __shared__ float rl2f[16];
@@ -237,7 +237,8 @@ template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
                                                double *__restrict__ partial, int assign) {
-    MFA_CUR(B);
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
     const double4 *__restrict__ pos = CUR(B.pos);
     const double4 *__restrict__ vel = CUR(B.vel);
     const int *__restrict__ cell_of = CUR(B.cell_of);
@@ -246,7 +247,6 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     int *rebuild = B.rebuild;
     __shared__ double tab[80];
     __shared__ double red[BLK];
-    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     const SysDev &P = sys[s];
     load_tables(P, tab);
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
@@ -414,7 +414,8 @@ template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
                                                double *__restrict__ partial, int assign) {
-    MFA_CUR(B);
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
     const double4 *__restrict__ pos = CUR(B.pos);
     const float4 *__restrict__ posf = CUR(B.posf);
     const float4 *__restrict__ velf = CUR(B.velf);
@@ -425,7 +426,6 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
     __shared__ float tf[96];
     __shared__ float4 off[128];         // cell offset (dx lx, dy ly, dz lz) per list code
     __shared__ double red[BLK];
-    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     const SysDev &P = sys[s];
     const int S = P.S;
     const float lx = (float)P.l_cell[0], ly = (float)P.l_cell[1], lz = (float)P.l_cell[2];

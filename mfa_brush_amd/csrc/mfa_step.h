@@ -14,8 +14,8 @@ namespace mfa {
 // ---------------------------------------------------------------------------
 template <bool MIXED>
 __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B) {
-    MFA_CUR(B);
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
     if (threadIdx.x == 0 && P.skin_mode == 0) { *B.rebuild = 1; B.chan_rb[s] = 1; }
@@ -174,8 +174,8 @@ template <int MODE, bool MIXED>
 __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                                double *__restrict__ partial, double *__restrict__ f_heads) {
     __shared__ double red[BLK];
-    MFA_CUR(B);
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
@@ -315,21 +315,13 @@ __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, con
                                             int *__restrict__ cell_count, int *__restrict__ bin_cell,
                                             int *__restrict__ bin_rank, int n_slots) {
     if (!*B.rebuild) return;
-    MFA_CUR(B);
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const int si = blk_sys[k / BLK];
         const SysDev &P = sys[si];
-        if (k - P.slot_base >= P.n_part) continue;
-        // A channel that does not rebuild is binned into its build-time cells:
-        // with the index order inside a cell this reproduces its layout
-        // exactly, so its results do not depend on the other channels.
-        int c;
-        if (B.chan_rb[si]) {
-            const double4 r = CUR(B.pos)[k];
-            c = cell_of_pos(P, r.x, r.y, r.z);
-        } else {
-            c = CUR(B.cell_of)[k];
-        }
+        if (k - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;   // only rebuilding channels
+        MFA_CUR(B, si);
+        const double4 r = CUR(B.pos)[k];
+        const int c = cell_of_pos(P, r.x, r.y, r.z);
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         bin_cell[k] = c;
         bin_rank[k] = atomicAdd(&cell_count[c], 1);
@@ -378,13 +370,21 @@ __global__ __launch_bounds__(BLK) void k_scan2(const int *__restrict__ rebuild, 
     }
 }
 
+// Cell ranges of the rebuilding channels: the global exclusive prefix counts
+// only their beads (k_bin skips the others), so a channel's ranges are prefix
+// differences from its first cell plus its slot base; the other channels keep
+// their ranges.
 __global__ __launch_bounds__(BLK) void k_scan3(const SysDev *__restrict__ sys, const int *__restrict__ rebuild,
-                                              int *__restrict__ cnt, const int *__restrict__ pre,
-                                              const int *__restrict__ bsum, const int *__restrict__ cell_sys,
-                                              int2 *__restrict__ crange, int n) {
+                                              const int *__restrict__ chan_rb, int *__restrict__ cnt,
+                                              const int *__restrict__ pre, const int *__restrict__ bsum,
+                                              const int *__restrict__ cell_sys, int2 *__restrict__ crange, int n) {
     if (!*rebuild) return;
     for (int c = blockIdx.x * BLK + threadIdx.x; c < n; c += gridDim.x * BLK) {
-        const int st = pre[c] + bsum[c / (4 * BLK)] + sys[cell_sys[c]].shift;
+        const int s = cell_sys[c];
+        if (!chan_rb[s]) continue;
+        const SysDev &P = sys[s];
+        const int cb = P.cell_base;
+        const int st = (pre[c] + bsum[c / (4 * BLK)]) - (pre[cb] + bsum[cb / (4 * BLK)]) + P.slot_base;
         crange[c] = make_int2(st, st + cnt[c]);
         cnt[c] = 0;
     }
@@ -396,8 +396,9 @@ __global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys,
                                                 int *__restrict__ perm, int n_slots) {
     if (!*B.rebuild) return;
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
-        const SysDev &P = sys[blk_sys[k / BLK]];
-        if (k - P.slot_base >= P.n_part) continue;
+        const int si = blk_sys[k / BLK];
+        const SysDev &P = sys[si];
+        if (k - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;
         const int c = bin_cell[k];
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         const int m = crange[c].x + bin_rank[k];
@@ -413,10 +414,11 @@ __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, co
                                              const int *__restrict__ bin_cell, const int2 *__restrict__ crange,
                                              const int *__restrict__ perm_tmp, int *__restrict__ perm, int n_slots) {
     if (!*B.rebuild) return;
-    MFA_CUR(B);
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
-        const SysDev &P = sys[blk_sys[k / BLK]];
-        if (k - P.slot_base >= P.n_part) continue;
+        const int si = blk_sys[k / BLK];
+        const SysDev &P = sys[si];
+        if (k - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;
+        MFA_CUR(B, si);
         const int c = bin_cell[k];
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         const int2 rg = crange[c];
@@ -438,11 +440,11 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
                                                 const int *__restrict__ perm, const int *__restrict__ bin_cell,
                                                 int n_slots) {
     if (!*B.rebuild) return;
-    MFA_CUR(B);
     for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
         const int si = blk_sys[m / BLK];
         const SysDev &P = sys[si];
-        if (m - P.slot_base >= P.n_part) continue;
+        if (m - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;
+        MFA_CUR(B, si);
         const int old = perm[m];
         if (!guard(B, in_range(old, n_slots), G_PERM, m)) continue;
         const double4 r = CUR(B.pos)[old];
@@ -451,9 +453,7 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         NXT(B.pos)[m] = r;
         NXT(B.vel)[m] = v;
         NXT(B.acc)[m] = CUR(B.acc)[old];
-        double4 rb = r;                                 // skin reference of the last build
-        if (!B.chan_rb[si]) rb = CUR(B.posb)[old];
-        NXT(B.posb)[m] = rb;
+        NXT(B.posb)[m] = r;                            // skin reference of this build
         const int cg = bin_cell[old];
         NXT(B.cell_of)[m] = cg;
         B.where[worig(r.w)] = m;
@@ -471,9 +471,10 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
     }
 }
 
-__global__ void k_flip(Bufs B, int *rebuild_count) {
+__global__ void k_flip(Bufs B, int *rebuild_count, int n_sys) {
     if (!*B.rebuild) return;
-    *B.parity ^= 1;
+    for (int s = 0; s < n_sys; ++s)
+        if (B.chan_rb[s]) B.parity[s] ^= 1;
     *rebuild_count += 1;
 }
 
@@ -483,7 +484,7 @@ __global__ void k_flip(Bufs B, int *rebuild_count) {
 __global__ __launch_bounds__(BLK) void k_upload(const SysDev *__restrict__ sys, int s, Bufs B, const double *__restrict__ r0,
                                                const double *__restrict__ v, const double *__restrict__ a,
                                                const int *__restrict__ type) {
-    MFA_CUR(B);
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= P.n_part) return;
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(BLK) void k_upload(const SysDev *__restrict__ sys, 
 
 template <int WHICH>
 __global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys, int s, Bufs B, double *__restrict__ out) {
-    MFA_CUR(B);
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= P.n_part) return;
@@ -524,7 +525,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 __global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, int s, Bufs B, double r_2_min,
                                               double *__restrict__ e_out, int *__restrict__ cnt_out,
                                               unsigned long long *__restrict__ hash_out) {
-    MFA_CUR(B);
+    MFA_CUR(B, s);
     __shared__ double4 tile[BLK];
     const SysDev &P = sys[s];
     const int i = blockIdx.x * BLK + threadIdx.x;
@@ -580,8 +581,8 @@ __global__ __launch_bounds__(BLK) void k_prof_bin(const SysDev *__restrict__ sys
                                                  int nb, unsigned int *__restrict__ cnt,
                                                  unsigned long long *__restrict__ vs,
                                                  unsigned long long *__restrict__ v2s) {
-    MFA_CUR(B);
     const int s = blk_sys[blockIdx.x];
+    MFA_CUR(B, s);
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
     if (k - P.slot_base >= P.n_part) return;

@@ -391,3 +391,35 @@ def test_error_paths():
         s.set_state(r0, v, t)
         with pytest.raises(Exception, match="inter_type"):
             s.fluid_wall(1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_melt_channel_vs_oracle(oracle_mod, precision):
+    """Melt without a brush (n_chain = 0): FENE + bending_melt + Poiseuille
+    body force + DPD, 10 fused steps against the oracle."""
+    p, t, r0, v = configs.channel_melt(n_cols=8, n_z=32, skin=0.3)
+    o = _oracle(oracle_mod, p)
+    a0 = np.zeros_like(r0)
+    ro, vo, ao, fo, eo, _ = o.md_steps(r0, v, a0, t, 0, 10, noise=1)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t, a0)
+        s.md_steps(10, graph=True)
+        st = s.get_state()
+        e = s.energies()
+    tol = 1e-9 if precision == "fp64" else 1e-4
+    _close(st["r0"], ro, tol)
+    _close(st["v"], vo, 1e3 * tol)
+    for k in ("v_fluid_fluid", "v_intra_molec", "v_bend_melt", "t_fluid"):
+        assert e[k] == pytest.approx(eo[k], rel=1e3 * tol), k
+
+
+def test_nve_drift_bounded():
+    """Thermostat off, 2000 steps of C1: total energy drift stays below
+    5e-4 of |E0| on both paths and the mixed path drifts like FP64."""
+    from mfa_brush_amd.nve import drift
+    p, t, r0, v = configs.channel_c1(thermostat=False, rc_dpd=None, skin=0.4)
+    p = p.replace(flags=Flags(**{**p.flags.__dict__, "thermostat": False}), friction=0.0, sig=0.0)
+    r64 = drift(p, t, r0, v, "fp64", 2000, 100)
+    r32 = drift(p, t, r0, v, "mixed", 2000, 100)
+    assert r64["max_abs_rel"] < 5e-4 and r32["max_abs_rel"] < 5e-4, (r64["max_abs_rel"], r32["max_abs_rel"])
+    assert abs(r32["slope_per_1000_steps"] - r64["slope_per_1000_steps"]) < 2e-5
