@@ -172,6 +172,15 @@ int  mfa_hip_set_neighbor_list(mfa_ctx *ctx, int32_t capacity);
 int  mfa_hip_get_neighbor_list(mfa_ctx *ctx, int32_t *capacity);
 int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* interacting pairs, last evaluation */
 
+/* DPD pairs each bead defers (0..16, default 16).  The pair kernels evaluate
+ * the LJ part of every pair while walking the partners and queue the DPD part
+ * (Philox normal, random and dissipative forces) of up to `depth` pairs per
+ * bead for a second pass with all lanes busy; pairs past the queue have their
+ * DPD part evaluated in place.  The forces are the same for every depth up
+ * to the FP64 summation order of the two parts.  No counterpart in the
+ * reference; a tuning and test knob of this implementation. */
+int  mfa_hip_set_dpd_queue(mfa_ctx *ctx, int32_t depth);
+
 const char *mfa_hip_last_error(void);
 int  mfa_hip_version(void);
 

@@ -478,6 +478,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     c->S_tot = (int)slots; c->N_tot = (int)parts; c->C_tot = (int)cells; c->H_tot = (int)heads;
     c->n_blocks = c->S_tot / BLK;
     c->B.n_slots = c->S_tot;
+    c->B.dpd_queue = DPD_Q;
     c->B.n_beads = c->N_tot;
     c->B.n_cells = c->C_tot;
     c->B.n_heads = std::max(1, c->H_tot);
@@ -936,6 +937,15 @@ int mfa_hip_get_neighbor_list(mfa_ctx *c, int32_t *capacity) {
     if (int rc = check_ctx(c, false)) return rc;
     if (!capacity) return fail(MFA_E_ARG, "get_neighbor_list: null");
     *capacity = c->B.nl_cap;
+    return 0;
+}
+
+int mfa_hip_set_dpd_queue(mfa_ctx *c, int32_t depth) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (depth < 0 || depth > DPD_Q) return fail(MFA_E_ARG, "set_dpd_queue: depth must be 0..16");
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->B.dpd_queue = depth;
+    invalidate_graph(c);
     return 0;
 }
 

@@ -68,6 +68,7 @@ struct Bufs {
     int *nl;            // Verlet list: entry q of slot k at nl[((q / 4) * n_slots + k) * 4 + q % 4]
     int *nl_n;          // list length per slot (> nl_cap: overflowed, the cells are scanned)
     int nl_cap;         // entries per slot; 0: no list, the pair kernels scan the cells
+    int dpd_queue;      // DPD pairs a pair-kernel lane defers (0..DPD_Q); the rest are evaluated in place
     int n_slots, n_beads, n_cells, n_heads;   // buffer extents, for the guards
 };
 

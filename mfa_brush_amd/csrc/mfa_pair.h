@@ -9,6 +9,18 @@
 // deterministic, and the pair forces conserve momentum up to the rounding of
 // those sums.
 //
+// Each pair is evaluated in two parts.  The LJ part runs while the bead walks
+// its partners.  A pair within the DPD range is queued in LDS, and its DPD
+// part (Philox normal, dissipative and random force) runs after the walk,
+// with every lane busy.  Only a few partners per bead are within the DPD
+// range (about 3 of 30 at a 1 sigma DPD cutoff), so evaluating the DPD part
+// in place would run the Philox rounds at nearly every step of the walk for
+// a few lanes of the wave.  Both parts are added into the FP64 accumulators
+// separately, in a fixed order: the LJ parts in walk order, then the queued
+// DPD parts in walk order.  A pair found when the queue is full has its DPD
+// part evaluated in place; the rule depends only on the walk order, so the
+// list and the cell scan give the same bits.
+//
 // Partners come from a Verlet list built when the bead's channel rebuilds its
 // cells (k_nlist): every bead of the 27 (or 125) neighbour cells closer than
 // the pair type's interaction range plus the skin.  The skin test (every
@@ -35,6 +47,8 @@ struct PairAcc {
 // the bead's cell
 constexpr int NL_CODE_BITS = 7;
 constexpr int NL_UNROLL = 4;      // partner loads issued together (one 16-byte list load)
+constexpr int DPD_Q = 16;         // LDS queue of deferred DPD pairs per lane
+constexpr int DPD_UNROLL = 2;     // deferred pairs whose partner loads are issued together
 // Entry q of slot k: groups of four entries, slots interleaved per group, so
 // a wave reading group q/4 of its 64 slots reads one contiguous 1 KB span and
 // a slot's four consecutive entries share one 16-byte word.
@@ -46,63 +60,69 @@ __device__ __forceinline__ int nl_code(int dx, int dy, int dz, int S) {
     return (dx + S) + w * ((dy + S) + w * (dz + S));
 }
 
-// One candidate j for bead i in FP64.  Returns nothing; accumulates into a.
+// LJ part of candidate pj for bead i in FP64 (fluid_fluid.f90:104-135), and
+// whether the pair takes a DPD term: 0 none, 1 within the LJ range, 2 beyond
+// it (DPD_CUT_OFF only).
 template <bool DPD, bool VIR>
-__device__ __forceinline__ void pair64(const SysDev &P, const double *tab, double xi, double yi, double zi,
-                                       int ti, int oi, double vxi, double vyi, double vzi,
-                                       double4 pj, const double4 *__restrict__ vel, int m,
-                                       double r_2_min, uint64_t t_idx, PairAcc &a) {
+__device__ __forceinline__ int pair64_lj(const SysDev &P, const double *tab, double xi, double yi, double zi,
+                                         int ti, double4 pj, double r_2_min, PairAcc &a) {
+#pragma clang fp contract(off)
     const int tj = wtype(pj.w);
     const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
-    double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
-    double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
-    double dz = zi - pj.z;
+    const double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
+    const double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
+    const double dz = zi - pj.z;
     double r_2 = r2_exact(dx, dy, dz);
-    const double r_cut2 = tab[32 + tt];
-    bool lj = r_2 < r_cut2;
-    double inv_sqrt = 0.0;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const bool lj = r_2 < tab[32 + tt];
     if (lj) {
         a.np += 1;
         if (P.force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
-        double inv_r_2 = 1.0 / r_2;
-        inv_sqrt = sqrt(inv_r_2);
-        double l_eps = tab[tt];
-        double r_61 = tab[16 + tt] * inv_r_2;
-        double r_6 = r_61 * r_61 * r_61;
-        double r_12 = r_6 * r_6;
-        double pot = (r_12 - r_6) - tab[48 + tt];
-        a.e += l_eps * pot;
-        double rd = l_eps * (-12.0 * r_12 + 6.0 * r_6) * inv_r_2;
-        fx = -rd * dx; fy = -rd * dy; fz = -rd * dz;
-    }
-    if (DPD) {
-        bool do_dpd;
-        if (P.dpd_cut_off) {
-            do_dpd = ti != 1 && tj != 1 && r_2 < P.r_cut_dpd_2;
-        } else {
-            do_dpd = lj;
-        }
-        if (do_dpd) {
-            double w;
-            if (P.dpd_cut_off) {
-                if (!lj) inv_sqrt = sqrt(1.0 / r_2);
-                w = 1.0 - sqrt(r_2 / P.r_cut_dpd_2);
-            } else {
-                w = 1.0 - sqrt(r_2 * tab[64 + tt]);
-            }
-            const int oj = worig(pj.w);
-            double g = (double)pair_gauss(P.seed, t_idx, oi - P.part_base, oj - P.part_base);
-            double rvx = dx * inv_sqrt, rvy = dy * inv_sqrt, rvz = dz * inv_sqrt;
-            double sr = P.sig * w * g;
-            double4 vj = vel[m];
-            double rdv = (vxi - vj.x) * rvx + (vyi - vj.y) * rvy + (vzi - vj.z) * rvz;
-            double sd = -1.0 * P.friction * (w * w) * rdv;
-            fx += sr * rvx + sd * rvx;
-            fy += sr * rvy + sd * rvy;
-            fz += sr * rvz + sd * rvz;
+        const double inv_r_2 = 1.0 / r_2;
+        const double l_eps = tab[tt];
+        const double r_61 = tab[16 + tt] * inv_r_2;
+        const double r_6 = r_61 * r_61 * r_61;
+        const double r_12 = r_6 * r_6;
+        a.e += l_eps * ((r_12 - r_6) - tab[48 + tt]);
+        const double rd = l_eps * (-12.0 * r_12 + 6.0 * r_6) * inv_r_2;
+        const double fx = -rd * dx, fy = -rd * dy, fz = -rd * dz;
+        a.fx += fx; a.fy += fy; a.fz += fz;
+        if (VIR) {
+            a.w[0] += dx * fx; a.w[1] += dy * fy; a.w[2] += dz * fz;
+            a.w[3] += dx * fy; a.w[4] += dx * fz; a.w[5] += dy * fz;
         }
     }
+    if (!DPD) return 0;
+    const bool do_dpd = P.dpd_cut_off ? (ti != 1 && tj != 1 && r_2 < P.r_cut_dpd_2) : lj;
+    return do_dpd ? (lj ? 1 : 2) : 0;
+}
+
+// DPD part of a pair that pair64_lj flagged (dpd_forces.f90:45-115).  lj: the
+// pair is within the LJ range, where the switch-on clamp of r^2 applies as in
+// the LJ part.
+template <bool VIR>
+__device__ __forceinline__ void pair64_dpd(const SysDev &P, const double *tab, double xi, double yi, double zi,
+                                           int ti, int oi, double vxi, double vyi, double vzi, double4 pj,
+                                           double4 vj, bool lj, double r_2_min, uint64_t t_idx, PairAcc &a) {
+#pragma clang fp contract(off)
+    const double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
+    const double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
+    const double dz = zi - pj.z;
+    double r_2 = r2_exact(dx, dy, dz);
+    if (lj && P.force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+    const double inv_sqrt = sqrt(1.0 / r_2);
+    double w;
+    if (P.dpd_cut_off) {
+        w = 1.0 - sqrt(r_2 / P.r_cut_dpd_2);
+    } else {
+        const int tt = ((ti - 1) & 3) + 4 * ((wtype(pj.w) - 1) & 3);
+        w = 1.0 - sqrt(r_2 * tab[64 + tt]);
+    }
+    const double g = (double)pair_gauss(P.seed, t_idx, oi - P.part_base, worig(pj.w) - P.part_base);
+    const double rvx = dx * inv_sqrt, rvy = dy * inv_sqrt, rvz = dz * inv_sqrt;
+    const double sr = P.sig * w * g;
+    const double rdv = (vxi - vj.x) * rvx + (vyi - vj.y) * rvy + (vzi - vj.z) * rvz;
+    const double sd = -1.0 * P.friction * (w * w) * rdv;
+    const double fx = sr * rvx + sd * rvx, fy = sr * rvy + sd * rvy, fz = sr * rvz + sd * rvz;
     a.fx += fx; a.fy += fy; a.fz += fz;
     if (VIR) {
         a.w[0] += dx * fx; a.w[1] += dy * fy; a.w[2] += dz * fz;
@@ -247,6 +267,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     int *rebuild = B.rebuild;
     __shared__ double tab[80];
     __shared__ double red[BLK];
+    __shared__ unsigned dq_buf[DPD_Q * BLK];   // deferred DPD pairs: slot | lj << 31, [j * BLK + lane]
     const SysDev &P = sys[s];
     load_tables(P, tab);
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
@@ -270,6 +291,23 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         double vxi = 0.0, vyi = 0.0, vzi = 0.0;
         if (DPD) { double4 v = vel[k]; vxi = v.x; vyi = v.y; vzi = v.z; }
+        unsigned *const myq = dq_buf + threadIdx.x;
+        const int dcap = B.dpd_queue;
+        int dq = 0;
+        // LJ part of candidate m (position pj); a DPD pair is queued, or
+        // evaluated in place when the queue is full
+        auto visit = [&](int m, double4 pj) {
+            const int f = pair64_lj<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, pj, r_2_min, a);
+            if (DPD && f) {
+                if (dq < dcap) {
+                    myq[dq * BLK] = (unsigned)m | (f == 1 ? 0x80000000u : 0u);
+                    ++dq;
+                } else {
+                    pair64_dpd<VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj, vel[m], f == 1, r_2_min,
+                                    (uint64_t)t_idx, a);
+                }
+            }
+        };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
             for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
@@ -282,9 +320,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                 for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[in_range(mm[u], B.n_slots) ? mm[u] : k];
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    if (q0 + u < nn && guard(B, in_range(mm[u], B.n_slots), G_SLOT, k))
-                        pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj[u], vel, mm[u], r_2_min,
-                                         (uint64_t)t_idx, a);
+                    if (q0 + u < nn && guard(B, in_range(mm[u], B.n_slots), G_SLOT, k)) visit(mm[u], pj[u]);
                 }
             }
         } else {
@@ -309,10 +345,30 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                         if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
                         for (int m = rg.x; m < rg.y; ++m) {
                             if (m == k) continue;
-                            pair64<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pos[m], vel, m,
-                                             r_2_min, (uint64_t)t_idx, a);
+                            visit(m, pos[m]);
                         }
                     }
+                }
+            }
+        }
+        // the queued DPD parts, in walk order
+        if (DPD) {
+            for (int j0 = 0; j0 < dq; j0 += DPD_UNROLL) {
+                unsigned e[DPD_UNROLL];
+                double4 pj[DPD_UNROLL], vj[DPD_UNROLL];
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) e[u] = myq[min(j0 + u, dq - 1) * BLK];
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) {
+                    const int m = (int)(e[u] & 0x7fffffffu);
+                    pj[u] = pos[m];
+                    vj[u] = vel[m];
+                }
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) {
+                    if (j0 + u < dq)
+                        pair64_dpd<VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj[u], vj[u],
+                                        (e[u] >> 31) != 0, r_2_min, (uint64_t)t_idx, a);
                 }
             }
         }
@@ -344,21 +400,37 @@ struct Pair32Ctx {
     uint64_t t_idx;
 };
 
-// One candidate m for bead k.  The FP32 separation is (x_k - x_m) - o, with o
-// the partner cell's offset (d * l_cell, exact for |d| <= 2): a difference of
-// the two cell-frame coordinates first, so the partner's evaluation computes
-// exactly the negated value.
+// Separation of candidate pj from bead pf in FP32 cell-frame coordinates, o
+// being the partner cell's offset (d * l_cell, exact for |d| <= 2): the
+// difference of the two cell-frame coordinates first, so the partner's
+// evaluation computes exactly the negated vector.  r^2 is written with
+// explicit fused multiply-adds (and contraction is off elsewhere in the pair
+// arithmetic), so every evaluation of a pair -- list walk, cell scan, queued
+// DPD part -- rounds the same way.
+struct Sep32 { float x, y, z, r2; };
+__device__ __forceinline__ Sep32 sep32(float4 pf, float4 pj, float ox, float oy, float oz) {
+#pragma clang fp contract(off)
+    Sep32 d;
+    d.x = (pf.x - pj.x) - ox;
+    d.y = (pf.y - pj.y) - oy;
+    d.z = (pf.z - pj.z) - oz;
+    d.r2 = __builtin_fmaf(d.z, d.z, __builtin_fmaf(d.y, d.y, d.x * d.x));
+    return d;
+}
+
+// LJ part of candidate pj (slot m) for bead k, and whether the pair takes a
+// DPD term: 0 none, 1 within the LJ range, 2 beyond it (DPD_CUT_OFF only).
 template <bool DPD, bool VIR>
-__device__ __forceinline__ void pair32(const SysDev &P, const float *tf, const Pair32Ctx &X,
-                                       const double4 *__restrict__ pos, const float4 *__restrict__ velf,
-                                       int k, int ti, int oi, float4 pf, float4 vi, float ox, float oy, float oz,
-                                       float4 pj, int m, PairAcc &a) {
-    const float ddx = (pf.x - pj.x) - ox, ddy = (pf.y - pj.y) - oy, ddz = (pf.z - pj.z) - oz;
-    float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+__device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const Pair32Ctx &X,
+                                         const double4 *__restrict__ pos, int k, int ti, float4 pf,
+                                         float ox, float oy, float oz, float4 pj, int m, PairAcc &a) {
+#pragma clang fp contract(off)
+    const Sep32 d = sep32(pf, pj, ox, oy, oz);
+    float r2 = d.r2;
     const int tj = fwtype(pj.w);
     const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
     const bool maybe_dpd = DPD && P.dpd_cut_off && r2 < X.rdpd2 * 1.0001f;
-    if (!(r2 < tf[48 + tt]) && !maybe_dpd) return;
+    if (!(r2 < tf[48 + tt]) && !maybe_dpd) return 0;
     bool lj = r2 < tf[32 + tt];
     if (!lj && r2 < tf[48 + tt]) {
         // borderline: decide with the FP64 reference arithmetic
@@ -368,45 +440,56 @@ __device__ __forceinline__ void pair32(const SysDev &P, const float *tf, const P
         double ez = pa.z - pb.z;
         lj = r2_exact(ex, ey, ez) < P.range_2[tt];
     }
-    float inv_sqrt = 0.f;
-    float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     if (lj) {
         a.np += 1;
         if (P.force_switch_on && r2 < X.r_2_min) r2 = X.r_2_min;
         const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
-        inv_sqrt = __builtin_amdgcn_rsqf(r2);
         const float l_eps = tf[tt];
         const float r_61 = tf[16 + tt] * inv_r_2;
         const float r_6 = r_61 * r_61 * r_61;
         const float r_12 = r_6 * r_6;
         a.e += (double)(l_eps * ((r_12 - r_6) - tf[64 + tt]));
-        const float rd = l_eps * (6.0f * r_6 - 12.0f * r_12) * inv_r_2;
-        pfx = -rd * ddx; pfy = -rd * ddy; pfz = -rd * ddz;
-    }
-    if (DPD) {
-        bool do_dpd;
-        float wgt;
-        if (P.dpd_cut_off) {
-            do_dpd = ti != 1 && tj != 1 && r2 < X.rdpd2;
-            if (do_dpd && !lj) inv_sqrt = __builtin_amdgcn_rsqf(r2);
-            wgt = 1.0f - r2 * inv_sqrt * X.inv_rdpd;
-        } else {
-            do_dpd = lj;
-            wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
-        }
-        if (do_dpd) {
-            const float g = pair_gauss(P.seed, X.t_idx, oi - P.part_base, fworig(pj.w) - P.part_base);
-            const float rvx = ddx * inv_sqrt, rvy = ddy * inv_sqrt, rvz = ddz * inv_sqrt;
-            const float4 vj = velf[m];
-            const float rdv = (vi.x - vj.x) * rvx + (vi.y - vj.y) * rvy + (vi.z - vj.z) * rvz;
-            const float sc = X.sigf * wgt * g - X.gamf * (wgt * wgt) * rdv;
-            pfx += sc * rvx; pfy += sc * rvy; pfz += sc * rvz;
+        const float rd = l_eps * __builtin_fmaf(6.0f, r_6, -12.0f * r_12) * inv_r_2;
+        const float fx = -rd * d.x, fy = -rd * d.y, fz = -rd * d.z;
+        a.fx += (double)fx; a.fy += (double)fy; a.fz += (double)fz;
+        if (VIR) {
+            a.w[0] += (double)(d.x * fx); a.w[1] += (double)(d.y * fy); a.w[2] += (double)(d.z * fz);
+            a.w[3] += (double)(d.x * fy); a.w[4] += (double)(d.x * fz); a.w[5] += (double)(d.y * fz);
         }
     }
-    a.fx += (double)pfx; a.fy += (double)pfy; a.fz += (double)pfz;
+    if (!DPD) return 0;
+    const bool do_dpd = P.dpd_cut_off ? (ti != 1 && tj != 1 && r2 < X.rdpd2) : lj;
+    return do_dpd ? (lj ? 1 : 2) : 0;
+}
+
+// DPD part of a pair that pair32_lj flagged (dpd_forces.f90:45-115).  lj: the
+// pair is within the LJ range, where the switch-on clamp of r^2 applies as in
+// the LJ part.
+template <bool VIR>
+__device__ __forceinline__ void pair32_dpd(const SysDev &P, const float *tf, const Pair32Ctx &X, int ti, int oi,
+                                           float4 pf, float4 vi, float ox, float oy, float oz, float4 pj,
+                                           float4 vj, bool lj, PairAcc &a) {
+#pragma clang fp contract(off)
+    const Sep32 d = sep32(pf, pj, ox, oy, oz);
+    float r2 = d.r2;
+    if (lj && P.force_switch_on && r2 < X.r_2_min) r2 = X.r_2_min;
+    const float inv_sqrt = __builtin_amdgcn_rsqf(r2);
+    float wgt;
+    if (P.dpd_cut_off) {
+        wgt = 1.0f - r2 * inv_sqrt * X.inv_rdpd;
+    } else {
+        const int tt = ((ti - 1) & 3) + 4 * ((fwtype(pj.w) - 1) & 3);
+        wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
+    }
+    const float g = pair_gauss(P.seed, X.t_idx, oi - P.part_base, fworig(pj.w) - P.part_base);
+    const float rvx = d.x * inv_sqrt, rvy = d.y * inv_sqrt, rvz = d.z * inv_sqrt;
+    const float rdv = __builtin_fmaf(vi.z - vj.z, rvz, __builtin_fmaf(vi.y - vj.y, rvy, (vi.x - vj.x) * rvx));
+    const float sc = __builtin_fmaf(X.sigf * wgt, g, -(X.gamf * (wgt * wgt) * rdv));
+    const float fx = sc * rvx, fy = sc * rvy, fz = sc * rvz;
+    a.fx += (double)fx; a.fy += (double)fy; a.fz += (double)fz;
     if (VIR) {
-        a.w[0] += (double)(ddx * pfx); a.w[1] += (double)(ddy * pfy); a.w[2] += (double)(ddz * pfz);
-        a.w[3] += (double)(ddx * pfy); a.w[4] += (double)(ddx * pfz); a.w[5] += (double)(ddy * pfz);
+        a.w[0] += (double)(d.x * fx); a.w[1] += (double)(d.y * fy); a.w[2] += (double)(d.z * fz);
+        a.w[3] += (double)(d.x * fy); a.w[4] += (double)(d.x * fz); a.w[5] += (double)(d.y * fz);
     }
 }
 
@@ -426,6 +509,8 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
     __shared__ float tf[96];
     __shared__ float4 off[128];         // cell offset (dx lx, dy ly, dz lz) per list code
     __shared__ double red[BLK];
+    __shared__ int dq_m[DPD_Q * BLK];              // deferred DPD pairs: partner slot, [j * BLK + lane]
+    __shared__ unsigned short dq_c[DPD_Q * BLK];   // and its cell-offset code | lj << 7
     const SysDev &P = sys[s];
     const int S = P.S;
     const float lx = (float)P.l_cell[0], ly = (float)P.l_cell[1], lz = (float)P.l_cell[2];
@@ -472,6 +557,25 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
         if (DPD) vi = velf[k];
+        int *const myq = dq_m + threadIdx.x;
+        unsigned short *const myc = dq_c + threadIdx.x;
+        const int dcap = B.dpd_queue;
+        int dq = 0;
+        // LJ part of candidate m (position pj, cell-offset code); a DPD pair
+        // is queued, or evaluated in place when the queue is full
+        auto visit = [&](int m, int code, float4 pj) {
+            const float4 o = off[code];
+            const int f = pair32_lj<DPD, VIR>(P, tf, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, a);
+            if (DPD && f) {
+                if (dq < dcap) {
+                    myq[dq * BLK] = m;
+                    myc[dq * BLK] = (unsigned short)(code | (f == 1 ? 0x80 : 0));
+                    ++dq;
+                } else {
+                    pair32_dpd<VIR>(P, tf, X, ti, oi, pf, vi, o.x, o.y, o.z, pj, velf[m], f == 1, a);
+                }
+            }
+        };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
             for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
@@ -487,10 +591,8 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
                     const int m = ent[u] >> NL_CODE_BITS;
-                    if (q0 + u < nn && guard(B, in_range(m, B.n_slots), G_SLOT, k)) {
-                        const float4 o = off[ent[u] & ((1 << NL_CODE_BITS) - 1)];
-                        pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, pj[u], m, a);
-                    }
+                    if (q0 + u < nn && guard(B, in_range(m, B.n_slots), G_SLOT, k))
+                        visit(m, ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
                 }
             }
         } else {
@@ -512,11 +614,36 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
                         x2 -= (x2 >= nx) ? nx : 0;
                         int2 rg = crange[row + x2];
                         if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
-                        const float4 o = off[nl_code(dx, dy, dz, S)];
+                        const int code = nl_code(dx, dy, dz, S);
                         for (int m = rg.x; m < rg.y; ++m) {
                             if (m == k) continue;
-                            pair32<DPD, VIR>(P, tf, X, pos, velf, k, ti, oi, pf, vi, o.x, o.y, o.z, posf[m], m, a);
+                            visit(m, code, posf[m]);
                         }
+                    }
+                }
+            }
+        }
+        // the queued DPD parts, in walk order
+        if (DPD) {
+            for (int j0 = 0; j0 < dq; j0 += DPD_UNROLL) {
+                int mm[DPD_UNROLL], cc[DPD_UNROLL];
+                float4 pj[DPD_UNROLL], vj[DPD_UNROLL];
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) {
+                    const int j = min(j0 + u, dq - 1);
+                    mm[u] = myq[j * BLK];
+                    cc[u] = myc[j * BLK];
+                }
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) {
+                    pj[u] = posf[mm[u]];
+                    vj[u] = velf[mm[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < DPD_UNROLL; ++u) {
+                    if (j0 + u < dq) {
+                        const float4 o = off[cc[u] & 0x7f];
+                        pair32_dpd<VIR>(P, tf, X, ti, oi, pf, vi, o.x, o.y, o.z, pj[u], vj[u], (cc[u] & 0x80) != 0, a);
                     }
                 }
             }

@@ -237,6 +237,40 @@ def test_neighbor_list_equals_cell_scan(precision):
         assert e["v_fluid_fluid"] == outs[0][1]["v_fluid_fluid"]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("rc_dpd", [1.0, None])
+def test_dpd_queue_depths_agree(oracle_mod, precision, rc_dpd):
+    """The DPD part of a pair runs from the per-lane queue or, once the queue
+    is full, in place.  Depths 16 (default) and 0 (all in place) differ only
+    by the FP64 summation order of the two parts; with a 2-deep queue (both
+    routes taken) the list and the cell scan agree bit for bit over steps
+    with rebuilds; at depth 0 the FP64 path matches the oracle."""
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=rc_dpd)
+    p = p.replace(skin=0.3)
+    fs = []
+    for depth in (16, 0):
+        with Simulation(p, precision=precision, t_idx0=17) as s:
+            s.set_dpd_queue(depth)
+            s.set_state(r0, v, t)
+            s.fluid_fluid()
+            fs.append(s.get_state()["force"])
+    _close(fs[1], fs[0], 1e-13)
+    if precision == "fp64":
+        fo, _ = _oracle(oracle_mod, p).fluid_fluid(r0, v, t, step=17, noise=1)
+        _close(fs[1], fo)
+    outs = []
+    for cap in (-1, 0):
+        with Simulation(p, precision=precision) as s:
+            s.set_dpd_queue(2)
+            s.set_neighbor_list(cap)
+            s.set_state(r0, v, t)
+            s.md_steps(20, graph=True)
+            outs.append((s.get_state(), s.flags()))
+    assert outs[0][1]["rebuilds"] > 1
+    for k in ("r0", "v", "a", "force"):
+        assert np.array_equal(outs[1][0][k], outs[0][0][k]), k
+
+
 def test_batched_replicas_equal_single_runs():
     ps = []
     states = []
