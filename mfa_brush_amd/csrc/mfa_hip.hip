@@ -60,7 +60,7 @@ struct mfa_ctx {
     Bufs B{};
     int *d_cell_count = nullptr, *d_pre = nullptr, *d_bsum = nullptr;
     int2 *d_crange = nullptr;
-    int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_perm_tmp = nullptr;
+    int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_key_tmp = nullptr, *d_bin_key = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
     int *d_rebuild_count = nullptr;
     double *d_stage = nullptr;
@@ -179,7 +179,7 @@ static void free_all(mfa_ctx *c) {
     F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.chan_rb); F(c->B.flags); F(c->B.t_idx);
     F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
-    F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_perm_tmp); F(c->d_partial);
+    F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -323,15 +323,16 @@ static void enqueue_rebuild(mfa_ctx *c) {
     const int g = std::min(GATED_GRID, std::max(1, c->n_blocks));
     const int gc = std::min(GATED_GRID, (c->C_tot + BLK - 1) / BLK);
     const int nsb = (c->C_tot + 4 * BLK - 1) / (4 * BLK);
-    LAUNCH(k_bin, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank, c->S_tot);
+    LAUNCH(k_bin, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_cell_count, c->d_bin_cell, c->d_bin_rank,
+           c->d_bin_key, c->S_tot);
     LAUNCH(k_scan1, nsb, c->st, c->B.rebuild, c->d_cell_count, c->C_tot, c->d_pre, c->d_bsum);
     LAUNCH(k_scan2, 1, c->st, c->B.rebuild, c->d_bsum, nsb);
     LAUNCH(k_scan3, gc, c->st, c->d_sys, c->B.rebuild, (const int *)c->B.chan_rb, c->d_cell_count, c->d_pre,
            c->d_bsum, c->d_cell_sys, c->d_crange, c->C_tot);
-    LAUNCH(k_scatter, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_bin_cell, c->d_bin_rank, c->d_crange,
-           c->d_perm_tmp, c->S_tot);
+    LAUNCH(k_scatter, g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_bin_cell, c->d_bin_rank, c->d_bin_key,
+           c->d_crange, c->d_key_tmp, c->S_tot);
     LAUNCH(k_rank, g, c->st, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
-           (const int *)c->d_perm_tmp, c->d_perm, c->S_tot);
+           (const int *)c->d_bin_key, (const int *)c->d_key_tmp, c->d_perm, c->S_tot);
     if (c->prec == MFA_PREC_MIXED)
         LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
     else
@@ -496,9 +497,14 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
         c->h_stage = nullptr; free_all(c); delete c; return fail(MFA_E_HIP, "pinned staging allocation failed");
     }
     // two halves of S_tot slots each (current / next, selected by parity)
-    ALLOC(c->B.pos, 2 * (size_t)c->S_tot); ALLOC(c->B.vel, 2 * (size_t)c->S_tot); ALLOC(c->B.acc, 2 * (size_t)c->S_tot);
+    // pos and posf carry NL_UNROLL - 1 zeroed entries past the upper half: the
+    // list build reads whole groups of four partners past a cell's end
+    ALLOC(c->B.pos, 2 * (size_t)c->S_tot + NL_UNROLL); ALLOC(c->B.vel, 2 * (size_t)c->S_tot);
+    ALLOC(c->B.acc, 2 * (size_t)c->S_tot);
     ALLOC(c->B.posb, 2 * (size_t)c->S_tot); ALLOC(c->B.cell_of, 2 * (size_t)c->S_tot);
-    if (precision == MFA_PREC_MIXED) { ALLOC(c->B.posf, 2 * (size_t)c->S_tot); ALLOC(c->B.velf, 2 * (size_t)c->S_tot); }
+    if (precision == MFA_PREC_MIXED) {
+        ALLOC(c->B.posf, 2 * (size_t)c->S_tot + NL_UNROLL); ALLOC(c->B.velf, 2 * (size_t)c->S_tot);
+    }
     ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, n_sys); ALLOC(c->B.rebuild, 1);
     ALLOC(c->B.chan_rb, n_sys);
     ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1);
@@ -507,7 +513,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_bsum, (c->C_tot + 4 * BLK - 1) / (4 * BLK) + 1);
     ALLOC(c->d_crange, c->C_tot);
     ALLOC(c->d_bin_cell, c->S_tot); ALLOC(c->d_bin_rank, c->S_tot); ALLOC(c->d_perm, c->S_tot);
-    ALLOC(c->d_perm_tmp, c->S_tot);
+    ALLOC(c->d_key_tmp, c->S_tot); ALLOC(c->d_bin_key, c->S_tot);
     ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
     ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);

@@ -173,11 +173,16 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     if (!B.chan_rb[s]) return;
     MFA_CUR(B, s);
     const SysDev &P = sys[s];
-    __shared__ double rl2d[16];
-    __shared__ float rl2f[16];
-    if (threadIdx.x < 16) {
-        rl2d[threadIdx.x] = P.rl2[threadIdx.x];
-        rl2f[threadIdx.x] = (float)(P.rl2[threadIdx.x] * (1.0 + 1.0 / 4096.0));
+    // list radius^2 per (bead type row, partner type bits 0..7); the FP32
+    // threshold is widened by 2^-12 and the FP64 one by 2^-40, so rounding
+    // can only add partners
+    __shared__ double rl2d[32];
+    __shared__ float rl2f[32];
+    if (threadIdx.x < 32) {
+        const int tr = threadIdx.x >> 3, tb = threadIdx.x & 7;
+        const double r = P.rl2[tr + 4 * ((tb - 1) & 3)];
+        rl2d[threadIdx.x] = r * (1.0 + 0x1p-40);
+        rl2f[threadIdx.x] = (float)(r * (1.0 + 1.0 / 4096.0));
     }
     __syncthreads();
     const int k = blockIdx.x * BLK + threadIdx.x;
@@ -191,13 +196,16 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int S = P.S;
     const int cap = B.nl_cap;
-    const size_t stride = (size_t)B.n_slots;
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
     float4 pfi = make_float4(0.f, 0.f, 0.f, 0.f);
     int ti;
     if (MIXED) { pfi = posf[k]; ti = fwtype(pfi.w); }
     else { pi = pos[k]; ti = wtype(pi.w); }
-    const int tr = (ti - 1) & 3;
+    const int trow = ((ti - 1) & 3) * 8;
+    // entry n of slot k is written through wp: one int further per entry, to
+    // the next group row after every fourth
+    int *wp = B.nl + (size_t)k * 4;
+    const size_t group_step = (size_t)B.n_slots * 4 - 3;
     int n = 0;
     for (int dz = -S; dz <= S; ++dz) {
         const int z2 = cz + dz;
@@ -213,32 +221,44 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                 if (x2 < 0) { x2 += nx; sx = -P.L[0]; } else if (x2 >= nx) { x2 -= nx; sx = P.L[0]; }
                 int2 rg = crange[row + x2];
                 if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
-                const int code = nl_code(dx, dy, dz, S);
-                const float ox = (float)dx * (float)P.l_cell[0], oy = (float)dy * (float)P.l_cell[1],
-                            oz = (float)dz * (float)P.l_cell[2];
-                // four partner loads in flight per iteration (indices clamped
-                // into the range; the clamped duplicates are not used)
+                const int ecode = nl_code(dx, dy, dz, S);
+                // the bead's position moved into the partner cell's frame
+                // (mixed: minus the cell offset; FP64: minus the periodic image)
+                const float qfx = pfi.x - (float)dx * (float)P.l_cell[0];
+                const float qfy = pfi.y - (float)dy * (float)P.l_cell[1];
+                const float qfz = pfi.z - (float)dz * (float)P.l_cell[2];
+                const double qx = pi.x - sx, qy = pi.y - sy;
+                // four partner loads in flight per iteration; loads past the
+                // cell's end stay inside the padded buffer and are not used
                 for (int m0 = rg.x; m0 < rg.y; m0 += NL_UNROLL) {
                     bool take[NL_UNROLL];
                     if (MIXED) {
                         float4 pj[NL_UNROLL];
-                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = posf[min(m0 + u, rg.y - 1)];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = posf[m0 + u];
+#pragma unroll
                         for (int u = 0; u < NL_UNROLL; ++u) {
-                            const float ex = (pfi.x - pj[u].x) - ox, ey = (pfi.y - pj[u].y) - oy, ez = (pfi.z - pj[u].z) - oz;
-                            take[u] = ex * ex + ey * ey + ez * ez < rl2f[tr + 4 * ((fwtype(pj[u].w) - 1) & 3)];
+                            const float ex = qfx - pj[u].x, ey = qfy - pj[u].y, ez = qfz - pj[u].z;
+                            take[u] = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <
+                                      rl2f[trow + (int)(__float_as_uint(pj[u].w) & 7u)];
                         }
                     } else {
                         double4 pj[NL_UNROLL];
-                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[min(m0 + u, rg.y - 1)];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[m0 + u];
+#pragma unroll
                         for (int u = 0; u < NL_UNROLL; ++u) {
-                            const double ex = pi.x - (pj[u].x + sx), ey = pi.y - (pj[u].y + sy), ez = pi.z - pj[u].z;
-                            take[u] = ex * ex + ey * ey + ez * ez < rl2d[tr + 4 * ((wtype(pj[u].w) - 1) & 3)];
+                            const double ex = qx - pj[u].x, ey = qy - pj[u].y, ez = pi.z - pj[u].z;
+                            take[u] = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) <
+                                      rl2d[trow + wtype(pj[u].w)];
                         }
                     }
+#pragma unroll
                     for (int u = 0; u < NL_UNROLL; ++u) {
                         const int m = m0 + u;
                         if (m < rg.y && m != k && take[u]) {
-                            if (n < cap) B.nl[nl_index(k, n, stride)] = (m << NL_CODE_BITS) | code;
+                            if (n < cap) *wp = (m << NL_CODE_BITS) | ecode;
+                            wp += ((n & 3) == 3) ? group_step : 1;
                             ++n;
                         }
                     }

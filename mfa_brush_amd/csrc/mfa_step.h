@@ -313,18 +313,39 @@ __global__ __launch_bounds__(BLK) void k_clear_rebuild(Bufs B, int n_sys) {
 
 __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                             int *__restrict__ cell_count, int *__restrict__ bin_cell,
-                                            int *__restrict__ bin_rank, int n_slots) {
+                                            int *__restrict__ bin_rank, int *__restrict__ bin_key, int n_slots) {
     if (!*B.rebuild) return;
-    for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
-        const int si = blk_sys[k / BLK];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int k0 = blockIdx.x * BLK; k0 < n_slots; k0 += gridDim.x * BLK) {
+        const int k = k0 + threadIdx.x;
+        const int si = blk_sys[k0 / BLK];
         const SysDev &P = sys[si];
-        if (k - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;   // only rebuilding channels
-        MFA_CUR(B, si);
-        const double4 r = CUR(B.pos)[k];
-        const int c = cell_of_pos(P, r.x, r.y, r.z);
-        if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
-        bin_cell[k] = c;
-        bin_rank[k] = atomicAdd(&cell_count[c], 1);
+        int c = -1;
+        if (k - P.slot_base < P.n_part && B.chan_rb[si]) {   // only rebuilding channels
+            MFA_CUR(B, si);
+            const double4 r = CUR(B.pos)[k];
+            const int cc = cell_of_pos(P, r.x, r.y, r.z);
+            if (guard(B, in_range(cc, B.n_cells), G_BIN, k)) {
+                c = cc;
+                bin_cell[k] = c;
+                bin_key[k] = worig(r.w);
+            }
+        }
+        // rank within the cell (arbitrary order; k_rank orders the cell):
+        // the beads stay sorted by cell between rebuilds, so a wave spans a
+        // few cells, and one atomic per distinct cell replaces one per bead
+        unsigned long long todo = __ballot(c >= 0);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int cl = __shfl(c, leader);
+            const unsigned long long peers = __ballot(c == cl);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&cell_count[cl], __popcll(peers));
+            base = __shfl(base, leader);
+            if (c == cl) bin_rank[k] = base + __popcll(peers & below);
+            todo &= ~peers;
+        }
     }
 }
 
@@ -392,8 +413,9 @@ __global__ __launch_bounds__(BLK) void k_scan3(const SysDev *__restrict__ sys, c
 
 __global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                 Bufs B, const int *__restrict__ bin_cell,
-                                                const int *__restrict__ bin_rank, const int2 *__restrict__ crange,
-                                                int *__restrict__ perm, int n_slots) {
+                                                const int *__restrict__ bin_rank, const int *__restrict__ bin_key,
+                                                const int2 *__restrict__ crange, int *__restrict__ key_tmp,
+                                                int n_slots) {
     if (!*B.rebuild) return;
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const int si = blk_sys[k / BLK];
@@ -403,35 +425,31 @@ __global__ __launch_bounds__(BLK) void k_scatter(const SysDev *__restrict__ sys,
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         const int m = crange[c].x + bin_rank[k];
         if (!guard(B, in_range(m, n_slots), G_RANGE, k)) continue;
-        perm[m] = k;
+        key_tmp[m] = bin_key[k];
     }
 }
 
 // Order each cell by reference bead index (the atomic ranks of k_bin are
-// arbitrary): every bead counts the beads of its cell with a smaller index
-// and takes that slot.  perm_tmp holds the atomic order, perm the final one.
+// arbitrary): every bead counts the beads of its cell with a smaller index,
+// from the cell's keys that k_scatter laid out contiguously, and takes that
+// slot.
 __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                              const int *__restrict__ bin_cell, const int2 *__restrict__ crange,
-                                             const int *__restrict__ perm_tmp, int *__restrict__ perm, int n_slots) {
+                                             const int *__restrict__ bin_key, const int *__restrict__ key_tmp,
+                                             int *__restrict__ perm, int n_slots) {
     if (!*B.rebuild) return;
     for (int k = blockIdx.x * BLK + threadIdx.x; k < n_slots; k += gridDim.x * BLK) {
         const int si = blk_sys[k / BLK];
         const SysDev &P = sys[si];
         if (k - P.slot_base >= P.n_part || !B.chan_rb[si]) continue;
-        MFA_CUR(B, si);
         const int c = bin_cell[k];
         if (!guard(B, in_range(c, B.n_cells), G_BIN, k)) continue;
         const int2 rg = crange[c];
         if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= n_slots, G_RANGE, k)) continue;
-        const int key = worig(CUR(B.pos)[k].w);
+        const int key = bin_key[k];
         int rank = 0;
-        bool ok = true;
-        for (int m = rg.x; m < rg.y; ++m) {
-            const int q = perm_tmp[m];
-            if (!guard(B, in_range(q, n_slots), G_PERM, m)) { ok = false; break; }
-            rank += worig(CUR(B.pos)[q].w) < key ? 1 : 0;
-        }
-        if (ok && guard(B, rank < rg.y - rg.x, G_RANGE, k)) perm[rg.x + rank] = k;
+        for (int m = rg.x; m < rg.y; ++m) rank += key_tmp[m] < key ? 1 : 0;
+        if (guard(B, rank < rg.y - rg.x, G_RANGE, k)) perm[rg.x + rank] = k;
     }
 }
 
