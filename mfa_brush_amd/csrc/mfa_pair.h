@@ -202,10 +202,11 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     if (MIXED) { pfi = posf[k]; ti = fwtype(pfi.w); }
     else { pi = pos[k]; ti = wtype(pi.w); }
     const int trow = ((ti - 1) & 3) * 8;
-    // entry n of slot k is written through wp: one int further per entry, to
-    // the next group row after every fourth
-    int *wp = B.nl + (size_t)k * 4;
-    const size_t group_step = (size_t)B.n_slots * 4 - 3;
+    // entries are collected four at a time in registers (b0..b3, oldest
+    // first) and written as one 16-byte group: a complete chunk per store
+    int4 *gp = (int4 *)B.nl + k;                // group n / 4 of slot k
+    const size_t group_step = (size_t)B.n_slots;
+    int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     int n = 0;
     for (int dz = -S; dz <= S; ++dz) {
         const int z2 = cz + dz;
@@ -257,8 +258,11 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                     for (int u = 0; u < NL_UNROLL; ++u) {
                         const int m = m0 + u;
                         if (m < rg.y && m != k && take[u]) {
-                            if (n < cap) *wp = (m << NL_CODE_BITS) | ecode;
-                            wp += ((n & 3) == 3) ? group_step : 1;
+                            b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
+                            if ((n & 3) == 3) {
+                                if (n < cap) *gp = make_int4(b0, b1, b2, b3);
+                                gp += group_step;
+                            }
                             ++n;
                         }
                     }
@@ -266,6 +270,10 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
             }
         }
     }
+    // the last, partial group (the pair kernels read only its first n % 4)
+    const int r = n & 3;
+    if (r != 0 && n <= cap) *gp = r == 1 ? make_int4(b3, 0, 0, 0) : r == 2 ? make_int4(b2, b3, 0, 0)
+                                                                  : make_int4(b1, b2, b3, 0);
     B.nl_n[k] = n;
     if (n > cap) atomicAdd(&B.flags[4], 1);
 }
