@@ -62,7 +62,7 @@ struct mfa_ctx {
     int2 *d_crange = nullptr;
     int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_key_tmp = nullptr, *d_bin_key = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
-    int *d_rebuild_count = nullptr;
+    int *d_rebuild_count = nullptr, *d_permute_done = nullptr;
     double *d_stage = nullptr;
     int *d_istage = nullptr;
     char *h_stage = nullptr;      // pinned staging buffer for every host<->device copy
@@ -180,7 +180,7 @@ static void free_all(mfa_ctx *c) {
     F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
-    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_stage); F(c->d_istage);
+    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -334,10 +334,11 @@ static void enqueue_rebuild(mfa_ctx *c) {
     LAUNCH(k_rank, g, c->st, c->d_sys, c->d_blk_sys, c->B, (const int *)c->d_bin_cell, (const int2 *)c->d_crange,
            (const int *)c->d_bin_key, (const int *)c->d_key_tmp, c->d_perm, c->S_tot);
     if (c->prec == MFA_PREC_MIXED)
-        LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
+        LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot,
+               c->d_permute_done, c->d_rebuild_count, c->n_sys);
     else
-        LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot);
-    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, c->st, c->B, c->d_rebuild_count, c->n_sys);
+        LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot,
+               c->d_permute_done, c->d_rebuild_count, c->n_sys);
     if (c->B.nl_cap > 0) {
         if (c->prec == MFA_PREC_MIXED)
             LAUNCH((k_nlist<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange);
@@ -515,7 +516,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_bin_cell, c->S_tot); ALLOC(c->d_bin_rank, c->S_tot); ALLOC(c->d_perm, c->S_tot);
     ALLOC(c->d_key_tmp, c->S_tot); ALLOC(c->d_bin_key, c->S_tot);
     ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
-    ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1);
+    ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1); ALLOC(c->d_permute_done, 1);
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
 #undef ALLOC
     if ((rc = alloc_list(c))) { free_all(c); delete c; return rc; }

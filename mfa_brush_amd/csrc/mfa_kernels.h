@@ -45,7 +45,7 @@ struct SysDev {
 };
 
 // Double-buffered bead arrays: a rebuild permutes a channel into the other
-// half and a one-thread kernel flips that channel's `parity`, so no copy-back
+// half and the permute kernel's last block flips that channel's `parity`, so no copy-back
 // pass is needed, channels that do not rebuild are not touched, and the host
 // never has to know whether the device decided to rebuild.  Each array is one
 // allocation of 2 * n_slots entries; channel s's current half starts at

@@ -32,6 +32,8 @@
 //
 // Arithmetic follows fluid_fluid.f90:97-154 and dpd_forces.f90:45-115.
 #pragma once
+#include <type_traits>
+
 #include "mfa_kernels.h"
 
 namespace mfa {
@@ -47,6 +49,7 @@ struct PairAcc {
 // the bead's cell
 constexpr int NL_CODE_BITS = 7;
 constexpr int NL_UNROLL = 4;      // partner loads issued together (one 16-byte list load)
+constexpr int NL_STAGE = 512;     // partner beads a list-build block stages in LDS per row of cells
 constexpr int DPD_Q = 16;         // LDS queue of deferred DPD pairs per lane
 constexpr int DPD_UNROLL = 2;     // deferred pairs whose partner loads are issued together
 // Entry q of slot k: groups of four entries, slots interleaved per group, so
@@ -173,34 +176,48 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     if (!B.chan_rb[s]) return;
     MFA_CUR(B, s);
     const SysDev &P = sys[s];
+    using Bead = typename std::conditional<MIXED, float4, double4>::type;
     // list radius^2 per (bead type row, partner type bits 0..7); the FP32
     // threshold is widened by 2^-12 and the FP64 one by 2^-40, so rounding
     // can only add partners
     __shared__ double rl2d[32];
     __shared__ float rl2f[32];
+    // the block's staged partners: one row of cells at a time
+    __shared__ alignas(16) unsigned char stage_raw[(NL_STAGE + NL_UNROLL) * sizeof(Bead)];
+    Bead *const stage = (Bead *)stage_raw;
     if (threadIdx.x < 32) {
         const int tr = threadIdx.x >> 3, tb = threadIdx.x & 7;
         const double r = P.rl2[tr + 4 * ((tb - 1) & 3)];
         rl2d[threadIdx.x] = r * (1.0 + 0x1p-40);
         rl2f[threadIdx.x] = (float)(r * (1.0 + 1.0 / 4096.0));
     }
-    __syncthreads();
-    const int k = blockIdx.x * BLK + threadIdx.x;
-    if (k - P.slot_base >= P.n_part) return;
-    const double4 *__restrict__ pos = CUR(B.pos);
-    const float4 *__restrict__ posf = CUR(B.posf);
+    const Bead *__restrict__ src;
+    if constexpr (MIXED) src = CUR(B.posf); else src = CUR(B.pos);
     const int *__restrict__ cell_of = CUR(B.cell_of);
-    int c = cell_of[k] - P.cell_base;
-    if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
     const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
-    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int S = P.S;
     const int cap = B.nl_cap;
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    const bool act = k - P.slot_base < P.n_part;
+    // the block's first and last cells (slots are sorted by cell)
+    const int k_last = min((int)(blockIdx.x * BLK + BLK - 1), P.slot_base + P.n_part - 1);
+    int c_lo = cell_of[blockIdx.x * BLK] - P.cell_base, c_hi = cell_of[k_last] - P.cell_base;
+    if (!in_range(c_lo, P.n_cells_tot) || !in_range(c_hi, P.n_cells_tot)) c_lo = c_hi = 0;
+    const int x_lo = c_lo % nx, y_lo = (c_lo / nx) % ny, z_lo = c_lo / (nx * ny);
+    const int x_hi = c_hi % nx, yz_lo = c_lo / nx, yz_hi = c_hi / nx;
+    int c = 0;
+    if (act) {
+        c = cell_of[k] - P.cell_base;
+        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
+    }
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
     float4 pfi = make_float4(0.f, 0.f, 0.f, 0.f);
-    int ti;
-    if (MIXED) { pfi = posf[k]; ti = fwtype(pfi.w); }
-    else { pi = pos[k]; ti = wtype(pi.w); }
+    int ti = 1;
+    if (act) {
+        if constexpr (MIXED) { pfi = src[k]; ti = fwtype(pfi.w); }
+        else { pi = src[k]; ti = wtype(pi.w); }
+    }
     const int trow = ((ti - 1) & 3) * 8;
     // entries are collected four at a time in registers (b0..b3, oldest
     // first) and written as one 16-byte group: a complete chunk per store
@@ -209,9 +226,41 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     int n = 0;
     for (int dz = -S; dz <= S; ++dz) {
-        const int z2 = cz + dz;
-        if (z2 < 0 || z2 >= nz) continue;
         for (int dy = -S; dy <= S; ++dy) {
+            // stage the partner cells of the block's cells for this (dy, dz),
+            // one contiguous slot range: without a periodic wrap in y the
+            // shifted cells of c_lo .. c_hi are c_lo + o .. c_hi + o, o = (dy +
+            // ny dz) nx, taken with S more cells on each side; with a wrap,
+            // the cells x_lo - S .. x_hi + S of the first row (to its end if
+            // the block runs into the next row).  Cells outside the stage are
+            // read from memory.
+            // (block-uniform: every thread computes the same range)
+            int lo = 0, cnt = 0, ca = 0, cb = -1;
+            {
+                const int zz = z_lo + dz;
+                const int ya = y_lo + dy, yb = (c_hi / nx) % ny + dy;
+                if (zz >= 0 && zz < nz) {
+                    if (ya >= 0 && yb < ny && yb >= ya) {
+                        const int o = (dy + ny * dz) * nx;
+                        ca = max(0, c_lo + o - S);
+                        cb = min(P.n_cells_tot - 1, c_hi + o + S);
+                    } else {
+                        const int yy = ya < 0 ? ya + ny : (ya >= ny ? ya - ny : ya);
+                        const int rowc = nx * (yy + ny * zz);
+                        ca = rowc + max(0, x_lo - S);
+                        cb = rowc + ((yz_hi == yz_lo) ? min(nx - 1, x_hi + S) : nx - 1);
+                    }
+                }
+                if (cb >= ca) {
+                    const int a = crange[P.cell_base + ca].x, b = crange[P.cell_base + cb].y;
+                    if (a >= 0 && b <= B.n_slots && b > a) { lo = a; cnt = min(b - a, NL_STAGE); }
+                }
+            }
+            __syncthreads();                 // the previous row's stage is no longer read
+            for (int i = threadIdx.x; i < cnt; i += BLK) stage[i] = src[lo + i];
+            __syncthreads();
+            const int z2 = cz + dz;
+            if (!act || z2 < 0 || z2 >= nz) continue;
             int y2 = cy + dy;
             double sy = 0.0;                 // periodic image of the partner row
             if (y2 < 0) { y2 += ny; sy = -P.L[1]; } else if (y2 >= ny) { y2 -= ny; sy = P.L[1]; }
@@ -229,47 +278,47 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                 const float qfy = pfi.y - (float)dy * (float)P.l_cell[1];
                 const float qfz = pfi.z - (float)dz * (float)P.l_cell[2];
                 const double qx = pi.x - sx, qy = pi.y - sy;
-                // four partner loads in flight per iteration; loads past the
-                // cell's end stay inside the padded buffer and are not used
-                for (int m0 = rg.x; m0 < rg.y; m0 += NL_UNROLL) {
-                    bool take[NL_UNROLL];
-                    if (MIXED) {
-                        float4 pj[NL_UNROLL];
+                // partners from the stage when the cell lies inside it, else
+                // from memory (padded buffer); four loads in flight, loads past
+                // the cell's end are not used
+                auto scan = [&](auto load) {
+                    for (int m0 = rg.x; m0 < rg.y; m0 += NL_UNROLL) {
+                        bool take[NL_UNROLL];
+                        Bead pj[NL_UNROLL];
 #pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = posf[m0 + u];
-#pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) {
-                            const float ex = qfx - pj[u].x, ey = qfy - pj[u].y, ez = qfz - pj[u].z;
-                            take[u] = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <
-                                      rl2f[trow + (int)(__float_as_uint(pj[u].w) & 7u)];
-                        }
-                    } else {
-                        double4 pj[NL_UNROLL];
-#pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[m0 + u];
+                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = load(m0 + u);
 #pragma unroll
                         for (int u = 0; u < NL_UNROLL; ++u) {
-                            const double ex = qx - pj[u].x, ey = qy - pj[u].y, ez = pi.z - pj[u].z;
-                            take[u] = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) <
-                                      rl2d[trow + wtype(pj[u].w)];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < NL_UNROLL; ++u) {
-                        const int m = m0 + u;
-                        if (m < rg.y && m != k && take[u]) {
-                            b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
-                            if ((n & 3) == 3) {
-                                if (n < cap) *gp = make_int4(b0, b1, b2, b3);
-                                gp += group_step;
+                            if constexpr (MIXED) {
+                                const float ex = qfx - pj[u].x, ey = qfy - pj[u].y, ez = qfz - pj[u].z;
+                                take[u] = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <
+                                          rl2f[trow + (int)(__float_as_uint(pj[u].w) & 7u)];
+                            } else {
+                                const double ex = qx - pj[u].x, ey = qy - pj[u].y, ez = pi.z - pj[u].z;
+                                take[u] = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) <
+                                          rl2d[trow + wtype(pj[u].w)];
                             }
-                            ++n;
+                        }
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; ++u) {
+                            const int m = m0 + u;
+                            if (m < rg.y && m != k && take[u]) {
+                                b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
+                                if ((n & 3) == 3) {
+                                    if (n < cap) *gp = make_int4(b0, b1, b2, b3);
+                                    gp += group_step;
+                                }
+                                ++n;
+                            }
                         }
                     }
-                }
+                };
+                if (rg.x >= lo && rg.y <= lo + cnt) scan([&](int m) { return stage[m - lo]; });
+                else scan([&](int m) { return src[m]; });
             }
         }
     }
+    if (!act) return;
     // the last, partial group (the pair kernels read only its first n % 4)
     const int r = n & 3;
     if (r != 0 && n <= cap) *gp = r == 1 ? make_int4(b3, 0, 0, 0) : r == 2 ? make_int4(b2, b3, 0, 0)

@@ -453,10 +453,14 @@ __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, co
     }
 }
 
+// Permutes the rebuilding channels into the other half.  The last block to
+// finish (every block has read the parities by then) flips those channels'
+// parities and counts the rebuild; `done` is its ticket counter, left at 0.
 template <bool MIXED>
 __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                                 const int *__restrict__ perm, const int *__restrict__ bin_cell,
-                                                int n_slots) {
+                                                int n_slots, int *__restrict__ done, int *__restrict__ rebuild_count,
+                                                int n_sys) {
     if (!*B.rebuild) return;
     for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
         const int si = blk_sys[m / BLK];
@@ -487,13 +491,18 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
             NXT(B.velf)[m] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
     }
-}
-
-__global__ void k_flip(Bufs B, int *rebuild_count, int n_sys) {
-    if (!*B.rebuild) return;
-    for (int s = 0; s < n_sys; ++s)
-        if (B.chan_rb[s]) B.parity[s] ^= 1;
-    *rebuild_count += 1;
+    // (no fence: the last block reads nothing the others wrote, and every
+    // block's parity reads were consumed before its ticket)
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        for (int s = 0; s < n_sys; ++s)
+            if (B.chan_rb[s]) B.parity[s] ^= 1;
+        *rebuild_count += 1;
+        *done = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
