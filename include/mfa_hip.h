@@ -172,7 +172,7 @@ int  mfa_hip_set_neighbor_list(mfa_ctx *ctx, int32_t capacity);
 int  mfa_hip_get_neighbor_list(mfa_ctx *ctx, int32_t *capacity);
 int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* interacting pairs, last evaluation */
 
-/* DPD pairs each bead defers (0..16, default 16).  The pair kernels evaluate
+/* DPD pairs each bead defers (0..8, default 8).  The pair kernels evaluate
  * the LJ part of every pair while walking the partners and queue the DPD part
  * (Philox normal, random and dissipative forces) of up to `depth` pairs per
  * bead for a second pass with all lanes busy; pairs past the queue have their

@@ -949,7 +949,7 @@ int mfa_hip_get_neighbor_list(mfa_ctx *c, int32_t *capacity) {
 
 int mfa_hip_set_dpd_queue(mfa_ctx *c, int32_t depth) {
     if (int rc = check_ctx(c, false)) return rc;
-    if (depth < 0 || depth > DPD_Q) return fail(MFA_E_ARG, "set_dpd_queue: depth must be 0..16");
+    if (depth < 0 || depth > DPD_Q) return fail(MFA_E_ARG, "set_dpd_queue: depth must be 0..8");
     HIPCHK(hipStreamSynchronize(c->st));
     c->B.dpd_queue = depth;
     invalidate_graph(c);

@@ -50,7 +50,7 @@ struct PairAcc {
 constexpr int NL_CODE_BITS = 7;
 constexpr int NL_UNROLL = 4;      // partner loads issued together (one 16-byte list load)
 constexpr int NL_STAGE = 512;     // partner beads a list-build block stages in LDS per row of cells
-constexpr int DPD_Q = 16;         // LDS queue of deferred DPD pairs per lane
+constexpr int DPD_Q = 8;          // LDS queue of deferred DPD pairs per lane
 constexpr int DPD_UNROLL = 2;     // deferred pairs whose partner loads are issued together
 // Entry q of slot k: groups of four entries, slots interleaved per group, so
 // a wave reading group q/4 of its 64 slots reads one contiguous 1 KB span and
@@ -63,23 +63,75 @@ __device__ __forceinline__ int nl_code(int dx, int dy, int dz, int S) {
     return (dx + S) + w * ((dy + S) + w * (dz + S));
 }
 
+// Slot range (lo, count <= cap) of the partner cells that a block of beads
+// in cells c_lo .. c_hi (channel-relative, sorted) has in row offset (dy, dz):
+// without a periodic wrap in y the shifted cells are c_lo + o .. c_hi + o,
+// o = (dy + ny dz) nx, taken with S more cells on each side, one contiguous
+// slot range; with a wrap, the cells x_lo - S .. x_hi + S of the first row (to
+// its end if the block runs into the next row).  Cells outside the range are
+// read from memory by the caller.  Block-uniform.
+__device__ __forceinline__ int2 stage_range(const SysDev &P, const Bufs &B, const int2 *__restrict__ crange,
+                                            int c_lo, int c_hi, int dy, int dz, int cap) {
+    const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2], S = P.S;
+    const int x_lo = c_lo % nx, y_lo = (c_lo / nx) % ny, z_lo = c_lo / (nx * ny);
+    const int x_hi = c_hi % nx, yz_lo = c_lo / nx, yz_hi = c_hi / nx;
+    int ca = 0, cb = -1;
+    const int zz = z_lo + dz;
+    const int ya = y_lo + dy, yb = (c_hi / nx) % ny + dy;
+    if (zz >= 0 && zz < nz) {
+        if (ya >= 0 && yb < ny && yb >= ya) {
+            const int o = (dy + ny * dz) * nx;
+            ca = max(0, c_lo + o - S);
+            cb = min(P.n_cells_tot - 1, c_hi + o + S);
+        } else {
+            const int yy = ya < 0 ? ya + ny : (ya >= ny ? ya - ny : ya);
+            const int rowc = nx * (yy + ny * zz);
+            ca = rowc + max(0, x_lo - S);
+            cb = rowc + ((yz_hi == yz_lo) ? min(nx - 1, x_hi + S) : nx - 1);
+        }
+    }
+    if (cb >= ca) {
+        const int a = crange[P.cell_base + ca].x, b = crange[P.cell_base + cb].y;
+        if (a >= 0 && b <= B.n_slots && b > a) return make_int2(a, min(b - a, cap));
+    }
+    return make_int2(0, 0);
+}
+
+// Channel scalars the FP64 pair arithmetic reads per candidate, copied out of
+// the channel record once per kernel: read through the record, the compiler
+// reloads them (scalar loads and waits) after every possible store.
+struct Pair64Ctx {
+    double L0, L1, iL0, iL1, rdpd2, sig, fric, r_2_min;
+    uint64_t seed, t_idx;
+    int part_base, dpd_cut, fso;
+};
+
+__device__ __forceinline__ Pair64Ctx pair64_ctx(const SysDev &P, long long t_idx, double r_2_min) {
+    Pair64Ctx C;
+    C.L0 = P.L[0]; C.L1 = P.L[1]; C.iL0 = P.invL[0]; C.iL1 = P.invL[1];
+    C.rdpd2 = P.r_cut_dpd_2; C.sig = P.sig; C.fric = P.friction; C.r_2_min = r_2_min;
+    C.seed = P.seed; C.t_idx = (uint64_t)t_idx;
+    C.part_base = P.part_base; C.dpd_cut = P.dpd_cut_off; C.fso = P.force_switch_on;
+    return C;
+}
+
 // LJ part of candidate pj for bead i in FP64 (fluid_fluid.f90:104-135), and
 // whether the pair takes a DPD term: 0 none, 1 within the LJ range, 2 beyond
 // it (DPD_CUT_OFF only).
 template <bool DPD, bool VIR>
-__device__ __forceinline__ int pair64_lj(const SysDev &P, const double *tab, double xi, double yi, double zi,
-                                         int ti, double4 pj, double r_2_min, PairAcc &a) {
+__device__ __forceinline__ int pair64_lj(const Pair64Ctx &C, const double *tab, double xi, double yi, double zi,
+                                         int ti, double4 pj, PairAcc &a) {
 #pragma clang fp contract(off)
     const int tj = wtype(pj.w);
     const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
-    const double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
-    const double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
+    const double dx = mic(xi - pj.x, C.L0, C.iL0);
+    const double dy = mic(yi - pj.y, C.L1, C.iL1);
     const double dz = zi - pj.z;
     double r_2 = r2_exact(dx, dy, dz);
     const bool lj = r_2 < tab[32 + tt];
     if (lj) {
         a.np += 1;
-        if (P.force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+        if (C.fso && r_2 < C.r_2_min) r_2 = C.r_2_min;
         const double inv_r_2 = 1.0 / r_2;
         const double l_eps = tab[tt];
         const double r_61 = tab[16 + tt] * inv_r_2;
@@ -95,7 +147,7 @@ __device__ __forceinline__ int pair64_lj(const SysDev &P, const double *tab, dou
         }
     }
     if (!DPD) return 0;
-    const bool do_dpd = P.dpd_cut_off ? (ti != 1 && tj != 1 && r_2 < P.r_cut_dpd_2) : lj;
+    const bool do_dpd = C.dpd_cut ? (ti != 1 && tj != 1 && r_2 < C.rdpd2) : lj;
     return do_dpd ? (lj ? 1 : 2) : 0;
 }
 
@@ -103,28 +155,28 @@ __device__ __forceinline__ int pair64_lj(const SysDev &P, const double *tab, dou
 // pair is within the LJ range, where the switch-on clamp of r^2 applies as in
 // the LJ part.
 template <bool VIR>
-__device__ __forceinline__ void pair64_dpd(const SysDev &P, const double *tab, double xi, double yi, double zi,
+__device__ __forceinline__ void pair64_dpd(const Pair64Ctx &C, const double *tab, double xi, double yi, double zi,
                                            int ti, int oi, double vxi, double vyi, double vzi, double4 pj,
-                                           double4 vj, bool lj, double r_2_min, uint64_t t_idx, PairAcc &a) {
+                                           double4 vj, bool lj, PairAcc &a) {
 #pragma clang fp contract(off)
-    const double dx = mic(xi - pj.x, P.L[0], P.invL[0]);
-    const double dy = mic(yi - pj.y, P.L[1], P.invL[1]);
+    const double dx = mic(xi - pj.x, C.L0, C.iL0);
+    const double dy = mic(yi - pj.y, C.L1, C.iL1);
     const double dz = zi - pj.z;
     double r_2 = r2_exact(dx, dy, dz);
-    if (lj && P.force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+    if (lj && C.fso && r_2 < C.r_2_min) r_2 = C.r_2_min;
     const double inv_sqrt = sqrt(1.0 / r_2);
     double w;
-    if (P.dpd_cut_off) {
-        w = 1.0 - sqrt(r_2 / P.r_cut_dpd_2);
+    if (C.dpd_cut) {
+        w = 1.0 - sqrt(r_2 / C.rdpd2);
     } else {
         const int tt = ((ti - 1) & 3) + 4 * ((wtype(pj.w) - 1) & 3);
         w = 1.0 - sqrt(r_2 * tab[64 + tt]);
     }
-    const double g = (double)pair_gauss(P.seed, t_idx, oi - P.part_base, worig(pj.w) - P.part_base);
+    const double g = (double)pair_gauss(C.seed, C.t_idx, oi - C.part_base, worig(pj.w) - C.part_base);
     const double rvx = dx * inv_sqrt, rvy = dy * inv_sqrt, rvz = dz * inv_sqrt;
-    const double sr = P.sig * w * g;
+    const double sr = C.sig * w * g;
     const double rdv = (vxi - vj.x) * rvx + (vyi - vj.y) * rvy + (vzi - vj.z) * rvz;
-    const double sd = -1.0 * P.friction * (w * w) * rdv;
+    const double sd = -1.0 * C.fric * (w * w) * rdv;
     const double fx = sr * rvx + sd * rvx, fy = sr * rvy + sd * rvy, fz = sr * rvz + sd * rvz;
     a.fx += fx; a.fy += fy; a.fz += fz;
     if (VIR) {
@@ -203,8 +255,6 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     const int k_last = min((int)(blockIdx.x * BLK + BLK - 1), P.slot_base + P.n_part - 1);
     int c_lo = cell_of[blockIdx.x * BLK] - P.cell_base, c_hi = cell_of[k_last] - P.cell_base;
     if (!in_range(c_lo, P.n_cells_tot) || !in_range(c_hi, P.n_cells_tot)) c_lo = c_hi = 0;
-    const int x_lo = c_lo % nx, y_lo = (c_lo / nx) % ny, z_lo = c_lo / (nx * ny);
-    const int x_hi = c_hi % nx, yz_lo = c_lo / nx, yz_hi = c_hi / nx;
     int c = 0;
     if (act) {
         c = cell_of[k] - P.cell_base;
@@ -227,35 +277,9 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     int n = 0;
     for (int dz = -S; dz <= S; ++dz) {
         for (int dy = -S; dy <= S; ++dy) {
-            // stage the partner cells of the block's cells for this (dy, dz),
-            // one contiguous slot range: without a periodic wrap in y the
-            // shifted cells of c_lo .. c_hi are c_lo + o .. c_hi + o, o = (dy +
-            // ny dz) nx, taken with S more cells on each side; with a wrap,
-            // the cells x_lo - S .. x_hi + S of the first row (to its end if
-            // the block runs into the next row).  Cells outside the stage are
-            // read from memory.
-            // (block-uniform: every thread computes the same range)
-            int lo = 0, cnt = 0, ca = 0, cb = -1;
-            {
-                const int zz = z_lo + dz;
-                const int ya = y_lo + dy, yb = (c_hi / nx) % ny + dy;
-                if (zz >= 0 && zz < nz) {
-                    if (ya >= 0 && yb < ny && yb >= ya) {
-                        const int o = (dy + ny * dz) * nx;
-                        ca = max(0, c_lo + o - S);
-                        cb = min(P.n_cells_tot - 1, c_hi + o + S);
-                    } else {
-                        const int yy = ya < 0 ? ya + ny : (ya >= ny ? ya - ny : ya);
-                        const int rowc = nx * (yy + ny * zz);
-                        ca = rowc + max(0, x_lo - S);
-                        cb = rowc + ((yz_hi == yz_lo) ? min(nx - 1, x_hi + S) : nx - 1);
-                    }
-                }
-                if (cb >= ca) {
-                    const int a = crange[P.cell_base + ca].x, b = crange[P.cell_base + cb].y;
-                    if (a >= 0 && b <= B.n_slots && b > a) { lo = a; cnt = min(b - a, NL_STAGE); }
-                }
-            }
+            // stage the partner cells of the block's cells for this (dy, dz)
+            const int2 st = stage_range(P, B, crange, c_lo, c_hi, dy, dz, NL_STAGE);
+            const int lo = st.x, cnt = st.y;
             __syncthreads();                 // the previous row's stage is no longer read
             for (int i = threadIdx.x; i < cnt; i += BLK) stage[i] = src[lo + i];
             __syncthreads();
@@ -358,6 +382,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         double r_time = (double)t_idx * P.dt;
         if (r_time < P.r_2_min_time) r_2_min = (1.0 - r_time / P.r_2_min_time) * P.r_2_min_init;
     }
+    const Pair64Ctx C = pair64_ctx(P, t_idx, r_2_min);
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
@@ -374,14 +399,13 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         // LJ part of candidate m (position pj); a DPD pair is queued, or
         // evaluated in place when the queue is full
         auto visit = [&](int m, double4 pj) {
-            const int f = pair64_lj<DPD, VIR>(P, tab, pi.x, pi.y, pi.z, ti, pj, r_2_min, a);
+            const int f = pair64_lj<DPD, VIR>(C, tab, pi.x, pi.y, pi.z, ti, pj, a);
             if (DPD && f) {
                 if (dq < dcap) {
                     myq[dq * BLK] = (unsigned)m | (f == 1 ? 0x80000000u : 0u);
                     ++dq;
                 } else {
-                    pair64_dpd<VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj, vel[m], f == 1, r_2_min,
-                                    (uint64_t)t_idx, a);
+                    pair64_dpd<VIR>(C, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj, vel[m], f == 1, a);
                 }
             }
         };
@@ -444,8 +468,8 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
 #pragma unroll
                 for (int u = 0; u < DPD_UNROLL; ++u) {
                     if (j0 + u < dq)
-                        pair64_dpd<VIR>(P, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj[u], vj[u],
-                                        (e[u] >> 31) != 0, r_2_min, (uint64_t)t_idx, a);
+                        pair64_dpd<VIR>(C, tab, pi.x, pi.y, pi.z, ti, oi, vxi, vyi, vzi, pj[u], vj[u],
+                                        (e[u] >> 31) != 0, a);
                 }
             }
         }
@@ -474,7 +498,8 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
 // ---------------------------------------------------------------------------
 struct Pair32Ctx {
     float r_2_min, rdpd2, inv_rdpd, sigf, gamf;
-    uint64_t t_idx;
+    uint64_t t_idx, seed;
+    int part_base, dpd_cut, fso;      // channel scalars, copied once (see Pair64Ctx)
 };
 
 // Separation of candidate pj from bead pf in FP32 cell-frame coordinates, o
@@ -506,7 +531,7 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
     float r2 = d.r2;
     const int tj = fwtype(pj.w);
     const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
-    const bool maybe_dpd = DPD && P.dpd_cut_off && r2 < X.rdpd2 * 1.0001f;
+    const bool maybe_dpd = DPD && X.dpd_cut && r2 < X.rdpd2 * 1.0001f;
     if (!(r2 < tf[48 + tt]) && !maybe_dpd) return 0;
     bool lj = r2 < tf[32 + tt];
     if (!lj && r2 < tf[48 + tt]) {
@@ -519,7 +544,7 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
     }
     if (lj) {
         a.np += 1;
-        if (P.force_switch_on && r2 < X.r_2_min) r2 = X.r_2_min;
+        if (X.fso && r2 < X.r_2_min) r2 = X.r_2_min;
         const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
         const float l_eps = tf[tt];
         const float r_61 = tf[16 + tt] * inv_r_2;
@@ -535,7 +560,7 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
         }
     }
     if (!DPD) return 0;
-    const bool do_dpd = P.dpd_cut_off ? (ti != 1 && tj != 1 && r2 < X.rdpd2) : lj;
+    const bool do_dpd = X.dpd_cut ? (ti != 1 && tj != 1 && r2 < X.rdpd2) : lj;
     return do_dpd ? (lj ? 1 : 2) : 0;
 }
 
@@ -549,16 +574,16 @@ __device__ __forceinline__ void pair32_dpd(const SysDev &P, const float *tf, con
 #pragma clang fp contract(off)
     const Sep32 d = sep32(pf, pj, ox, oy, oz);
     float r2 = d.r2;
-    if (lj && P.force_switch_on && r2 < X.r_2_min) r2 = X.r_2_min;
+    if (lj && X.fso && r2 < X.r_2_min) r2 = X.r_2_min;
     const float inv_sqrt = __builtin_amdgcn_rsqf(r2);
     float wgt;
-    if (P.dpd_cut_off) {
+    if (X.dpd_cut) {
         wgt = 1.0f - r2 * inv_sqrt * X.inv_rdpd;
     } else {
         const int tt = ((ti - 1) & 3) + 4 * ((fwtype(pj.w) - 1) & 3);
         wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
     }
-    const float g = pair_gauss(P.seed, X.t_idx, oi - P.part_base, fworig(pj.w) - P.part_base);
+    const float g = pair_gauss(X.seed, X.t_idx, oi - X.part_base, fworig(pj.w) - X.part_base);
     const float rvx = d.x * inv_sqrt, rvy = d.y * inv_sqrt, rvz = d.z * inv_sqrt;
     const float rdv = __builtin_fmaf(vi.z - vj.z, rvz, __builtin_fmaf(vi.y - vj.y, rvy, (vi.x - vj.x) * rvx));
     const float sc = __builtin_fmaf(X.sigf * wgt, g, -(X.gamf * (wgt * wgt) * rdv));
@@ -622,6 +647,7 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
         X.r_2_min = (float)r_2_min_d;
     }
     X.sigf = (float)P.sig; X.gamf = (float)P.friction;
+    X.seed = P.seed; X.part_base = P.part_base; X.dpd_cut = P.dpd_cut_off; X.fso = P.force_switch_on;
     X.rdpd2 = (float)P.r_cut_dpd_2;
     X.inv_rdpd = (float)(1.0 / sqrt(P.r_cut_dpd_2 > 0.0 ? P.r_cut_dpd_2 : 1.0));
     PairAcc a;

@@ -233,8 +233,8 @@ class Simulation:
         """Verlet-list capacity per bead: -1 automatic, 0 cell scan only."""
         self._c(self.lib.mfa_hip_set_neighbor_list(self.h, int(capacity)), "set_neighbor_list")
 
-    def set_dpd_queue(self, depth: int = 16):
-        """DPD pairs per bead whose DPD part is deferred to a second pass (0..16)."""
+    def set_dpd_queue(self, depth: int = 8):
+        """DPD pairs per bead whose DPD part is deferred to a second pass (0..8)."""
         self._c(self.lib.mfa_hip_set_dpd_queue(self.h, int(depth)), "set_dpd_queue")
 
     def neighbor_list(self) -> int:

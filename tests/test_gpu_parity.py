@@ -241,14 +241,14 @@ def test_neighbor_list_equals_cell_scan(precision):
 @pytest.mark.parametrize("rc_dpd", [1.0, None])
 def test_dpd_queue_depths_agree(oracle_mod, precision, rc_dpd):
     """The DPD part of a pair runs from the per-lane queue or, once the queue
-    is full, in place.  Depths 16 (default) and 0 (all in place) differ only
+    is full, in place.  Depths 8 (default) and 0 (all in place) differ only
     by the FP64 summation order of the two parts; with a 2-deep queue (both
     routes taken) the list and the cell scan agree bit for bit over steps
     with rebuilds; at depth 0 the FP64 path matches the oracle."""
     p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=rc_dpd)
     p = p.replace(skin=0.3)
     fs = []
-    for depth in (16, 0):
+    for depth in (8, 0):
         with Simulation(p, precision=precision, t_idx0=17) as s:
             s.set_dpd_queue(depth)
             s.set_state(r0, v, t)
