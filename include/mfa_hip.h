@@ -172,6 +172,15 @@ int  mfa_hip_set_neighbor_list(mfa_ctx *ctx, int32_t capacity);
 int  mfa_hip_get_neighbor_list(mfa_ctx *ctx, int32_t *capacity);
 int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* interacting pairs, last evaluation */
 
+/* Chain observables of channel isys from the device state (the gyration and
+ * momentum part of observation.f90:182-290): out[0..2] the brush chains'
+ * mean squared radius of gyration per axis (r_g2(1,:)), out[3..5] the melt
+ * chains' (r_g2(2,:)), out[6..8] the total momentum sum(mass * v) (tot_P
+ * before its running average).  Chains are unfolded across the periodic
+ * images as the reference does.  The anisotropy r_par_per2 follows from
+ * out[0..5] on the host. */
+int  mfa_hip_chain_observables(mfa_ctx *ctx, int32_t isys, double *out);
+
 /* DPD pairs each bead defers (0..8, default 8).  The pair kernels evaluate
  * the LJ part of every pair while walking the partners and queue the DPD part
  * (Philox normal, random and dissipative forces) of up to `depth` pairs per

@@ -63,6 +63,7 @@ struct mfa_ctx {
     int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_key_tmp = nullptr, *d_bin_key = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
     int *d_rebuild_count = nullptr, *d_permute_done = nullptr;
+    double *d_obs = nullptr;       // block partials of the chain observables (2 x channel blocks x 8)
     double *d_stage = nullptr;
     int *d_istage = nullptr;
     char *h_stage = nullptr;      // pinned staging buffer for every host<->device copy
@@ -180,7 +181,7 @@ static void free_all(mfa_ctx *c) {
     F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
-    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_stage); F(c->d_istage);
+    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_obs); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -517,6 +518,11 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_key_tmp, c->S_tot); ALLOC(c->d_bin_key, c->S_tot);
     ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
     ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1); ALLOC(c->d_permute_done, 1);
+    {
+        int mb = 1;
+        for (const SysDev &d : c->hs) mb = std::max(mb, (d.n_part + BLK - 1) / BLK + 1);
+        ALLOC(c->d_obs, 16 * (size_t)mb);
+    }
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
 #undef ALLOC
     if ((rc = alloc_list(c))) { free_all(c); delete c; return rc; }
@@ -769,6 +775,32 @@ int mfa_hip_get_f_on_heads(mfa_ctx *c, int32_t isys, double *f) {
     const SysDev &d = c->hs[isys];
     if (d.n_chain == 0) return 0;
     return to_host(c, f, c->d_fheads + 3 * (size_t)d.head_base, 3 * (size_t)d.n_chain * sizeof(double), "f_on_heads");
+}
+
+int mfa_hip_chain_observables(mfa_ctx *c, int32_t isys, double *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !out) return fail(MFA_E_ARG, "chain_observables: bad argument");
+    const SysDev &d = c->hs[isys];
+    const int nt = d.n_chain + d.n_chain_d;
+    const int gb = std::max(1, (nt + BLK - 1) / BLK), gm = std::max(1, (d.n_part + BLK - 1) / BLK);
+    // both partial sets fit: gb, gm <= blocks of the channel's slots
+    LAUNCH(k_chain_rg2, gb, c->st, c->d_sys, isys, c->B, c->d_obs);
+    LAUNCH(k_momentum, gm, c->st, c->d_sys, isys, c->B, c->d_obs + 8 * (size_t)gb);
+    std::vector<double> h(8 * (size_t)(gb + gm));
+    if (int rc = to_host(c, h.data(), c->d_obs, h.size() * sizeof(double), "chain observables")) return rc;
+    double rb[3] = {0, 0, 0}, rm[3] = {0, 0, 0}, pm[3] = {0, 0, 0};
+    for (int b = 0; b < gb; ++b)
+        for (int q = 0; q < 3; ++q) { rb[q] += h[8 * (size_t)b + q]; rm[q] += h[8 * (size_t)b + 3 + q]; }
+    for (int b = 0; b < gm; ++b)
+        for (int q = 0; q < 3; ++q) pm[q] += h[8 * (size_t)(gb + b) + q];
+    // observation.f90:256,272: mean over chains (brush times 1/n_chain, melt divided by n_chain_d)
+    const double inv_nc = d.n_chain > 0 ? 1.0 / (double)d.n_chain : 0.0;
+    for (int q = 0; q < 3; ++q) {
+        out[q] = rb[q] * inv_nc;
+        out[3 + q] = d.n_chain_d > 0 ? rm[q] / (double)d.n_chain_d : 0.0;
+        out[6 + q] = pm[q];
+    }
+    return 0;
 }
 
 int mfa_hip_get_flags(mfa_ctx *c, mfa_flags *out) {

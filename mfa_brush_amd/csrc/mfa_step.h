@@ -630,6 +630,93 @@ __global__ __launch_bounds__(BLK) void k_prof_bin(const SysDev *__restrict__ sys
     }
 }
 
+// ---------------------------------------------------------------------------
+// Chain observables of channel s (observation.f90:182-290).
+// ---------------------------------------------------------------------------
+// Radius of gyration per axis of every brush chain (n_mon beads, head first)
+// and melt chain (n_mon_d beads).  A chain is unfolded bead by bead against
+// its previous, already unfolded bead: a jump of more than half the box
+// along an axis moves the bead by one box length (:182-237).  Then the
+// centre of mass and the mean squared deviation per axis, summed in bead
+// order and normalised as the reference does (brush: times 1/n_mon; melt:
+// divided by n_mon_d) (:247-271).  Per block, in a fixed order: part[8 b +
+// 0..2] the sum over its brush chains, part[8 b + 3..5] over its melt chains.
+__device__ __forceinline__ void unfold_step(double *x, const double *prev, const double *L, const double *h) {
+#pragma clang fp contract(off)
+    for (int d = 0; d < 3; ++d) {
+        if (x[d] - prev[d] > h[d]) x[d] = x[d] - L[d];
+        else if (prev[d] - x[d] > h[d]) x[d] = x[d] + L[d];
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_chain_rg2(const SysDev *__restrict__ sys, int s, Bufs B,
+                                                  double *__restrict__ part) {
+#pragma clang fp contract(off)
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    __shared__ double red[BLK];
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    const int nb = P.n_chain, nt = P.n_chain + P.n_chain_d;
+    const bool brush = c < nb;
+    double g[3] = {0.0, 0.0, 0.0};
+    const int len = brush ? P.n_mon : P.n_mon_d;
+    if (c < nt && len > 0) {
+        const int first = P.part_base + (brush ? c * P.n_mon : P.pd + (c - nb) * P.n_mon_d);
+        const double L[3] = {P.L[0], P.L[1], P.L[2]};
+        const double h[3] = {P.L[0] / 2.0, P.L[1] / 2.0, P.L[2] / 2.0};
+        const double inv_len = 1.0 / (double)len;
+        auto bead = [&](int j, double *x) {
+            int k = B.where[first + j];
+            if (!guard(B, in_range(k, B.n_slots), G_WHERE, first + j)) k = 0;
+            const double4 r = CUR(B.pos)[k];
+            x[0] = r.x; x[1] = r.y; x[2] = r.z;
+        };
+        double sum[3] = {0.0, 0.0, 0.0}, prev[3] = {0.0, 0.0, 0.0}, x[3];
+        for (int j = 0; j < len; ++j) {
+            bead(j, x);
+            if (j > 0) unfold_step(x, prev, L, h);
+            for (int d = 0; d < 3; ++d) { sum[d] = sum[d] + x[d]; prev[d] = x[d]; }
+        }
+        double cm[3], acc[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; ++d) cm[d] = brush ? sum[d] * inv_len : sum[d] / (double)len;
+        for (int j = 0; j < len; ++j) {
+            bead(j, x);
+            if (j > 0) unfold_step(x, prev, L, h);
+            for (int d = 0; d < 3; ++d) {
+                const double e = x[d] - cm[d];
+                acc[d] = acc[d] + e * e;
+                prev[d] = x[d];
+            }
+        }
+        for (int d = 0; d < 3; ++d) g[d] = brush ? acc[d] * inv_len : acc[d] / (double)len;
+    }
+    for (int q = 0; q < 6; ++q) {
+        const bool mine = c < nt && ((q < 3) == brush);
+        const double t = block_sum(mine ? g[q % 3] : 0.0, red);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8 + q] = t;
+    }
+}
+
+// Total momentum of channel s, sum of mass * v over its beads
+// (observation.f90:286-290): per block of slots, part[8 b + 0..2].
+__global__ __launch_bounds__(BLK) void k_momentum(const SysDev *__restrict__ sys, int s, Bufs B,
+                                                 double *__restrict__ part) {
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    __shared__ double red[BLK];
+    const int k = P.slot_base + blockIdx.x * BLK + threadIdx.x;
+    double m[3] = {0.0, 0.0, 0.0};
+    if (k - P.slot_base < P.n_part) {
+        const double4 r = CUR(B.pos)[k], v = CUR(B.vel)[k];
+        const double ms = P.mass_type[(wtype(r.w) - 1) & 3];
+        m[0] = ms * v.x; m[1] = ms * v.y; m[2] = ms * v.z;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double t = block_sum(m[d], red);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8 + d] = t;
+    }
+}
+
 __global__ __launch_bounds__(BLK) void k_prof_fold(int n_cnt, int n_v, unsigned int *__restrict__ cnt,
                                                   unsigned long long *__restrict__ vs,
                                                   unsigned long long *__restrict__ v2s, double *__restrict__ dens,

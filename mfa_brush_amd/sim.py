@@ -180,6 +180,18 @@ class Simulation:
         self._c(self.lib.mfa_hip_get_f_on_heads(self.h, isys, _ptr(f)), "get_f_on_heads")
         return f
 
+    def chain_observables(self, isys: int = 0) -> dict:
+        """Gyration and momentum observables of observation.f90:182-290 from
+        the device state: mean squared radius of gyration per axis of the
+        brush and melt chains (r_g2), their anisotropy (r_par_per2) and the
+        total momentum (tot_P before its running average)."""
+        out = np.zeros(9)
+        self._c(self.lib.mfa_hip_chain_observables(self.h, isys, _ptr(out)), "chain_observables")
+        gb, gm = out[0:3], out[3:6]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            aniso = np.array([(g[2] - 0.5 * (g[0] + g[1])) / (g[0] + g[1] + g[2]) for g in (gb, gm)])
+        return dict(rg2_brush=gb, rg2_melt=gm, anisotropy=aniso, momentum=out[6:9])
+
     def flags(self) -> dict:
         f = _lib.Flags()
         self._c(self.lib.mfa_hip_get_flags(self.h, C.byref(f)), "get_flags")

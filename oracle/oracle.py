@@ -283,6 +283,55 @@ class Oracle:
 REF_DIR = os.path.join(HERE, "_ref")
 
 
+def chain_observables(p, r0, v, a_type):
+    """Gyration and momentum part of observation.f90:182-290 (numpy, bead order).
+
+    Chains are unfolded bead by bead against the previous, already unfolded
+    bead (:182-237); per chain the centre of mass and the mean squared
+    deviation per axis are summed in bead order and normalised as the
+    reference does, brush times 1/n_mon, melt divided by n_mon_d (:247-271);
+    the chain means are summed in chain order, brush times 1/n_chain, melt
+    divided by n_chain_d (:256, :272); the anisotropy is
+    (Rg2_z - (Rg2_x + Rg2_y)/2) / sum(Rg2) (:278-279); the momentum is
+    sum(mass * v) in bead order (:286-290).  Returns (rg2_brush[3],
+    rg2_melt[3], anisotropy[2], momentum[3])."""
+    r0 = np.asarray(r0, float)
+    L = np.array(p.boundary, float)
+    h = L / 2.0
+
+    def rg2(first, n_ch, n_m, brush):
+        if n_ch == 0 or n_m == 0:
+            return np.zeros(3)
+        X = r0[first:first + n_ch * n_m].reshape(n_ch, n_m, 3).copy()
+        for j in range(1, n_m):
+            x, prev = X[:, j, :], X[:, j - 1, :]
+            up = (x - prev) > h
+            dn = ~up & ((prev - x) > h)
+            X[:, j, :] = np.where(up, x - L, np.where(dn, x + L, x))
+        inv = 1.0 / float(n_m)
+        s = np.zeros((n_ch, 3))
+        for j in range(n_m):
+            s = s + X[:, j, :]
+        cm = s * inv if brush else s / float(n_m)
+        acc = np.zeros((n_ch, 3))
+        for j in range(n_m):
+            e = X[:, j, :] - cm
+            acc = acc + e * e
+        g = acc * inv if brush else acc / float(n_m)
+        tot = np.zeros(3)
+        for c in range(n_ch):
+            tot = tot + g[c]
+        return tot * (1.0 / float(n_ch)) if brush else tot / float(n_ch)
+
+    gb = rg2(0, p.n_chain, p.n_mon, True)
+    gm = rg2(p.n_chain * p.n_mon, p.n_chain_d, p.n_mon_d, False)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        aniso = np.array([(g[2] - 0.5 * (g[0] + g[1])) / (g[0] + g[1] + g[2]) for g in (gb, gm)])
+    m = np.asarray(p.mass_type, float)[np.asarray(a_type) - 1]
+    mom = np.cumsum(m[:, None] * np.asarray(v, float), axis=0)[-1]
+    return gb, gm, aniso, mom
+
+
 def ref_available(omp: bool = False) -> bool:
     return os.path.exists(os.path.join(REF_DIR, "libmfa_ref_omp.so" if omp else "libmfa_ref.so"))
 

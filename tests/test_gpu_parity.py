@@ -457,3 +457,28 @@ def test_nve_drift_bounded():
     r32 = drift(p, t, r0, v, "mixed", 2000, 100)
     assert r64["max_abs_rel"] < 5e-4 and r32["max_abs_rel"] < 5e-4, (r64["max_abs_rel"], r32["max_abs_rel"])
     assert abs(r32["slope_per_1000_steps"] - r64["slope_per_1000_steps"]) < 2e-5
+
+
+@pytest.mark.parametrize("system", ["brush_solvent", "brush_melt"])
+def test_chain_observables_vs_oracle(system):
+    """Radius of gyration per axis of the brush and melt chains (unfolded
+    across the periodic images), anisotropy and total momentum from the
+    device state match the oracle's restatement of observation.f90:182-290
+    on the same downloaded state, after steps that move chains across the
+    periodic boundaries."""
+    from oracle.oracle import chain_observables
+    if system == "brush_solvent":
+        p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    else:
+        p, t, r0, v = _chain_system()
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(30, graph=True)
+        st = s.get_state()
+        got = s.chain_observables()
+    gb, gm, an, mom = chain_observables(p, st["r0"], st["v"], t)
+    assert np.allclose(got["rg2_brush"], gb, rtol=1e-12, atol=0)
+    assert np.allclose(got["rg2_melt"], gm, rtol=1e-12, atol=1e-300)
+    assert np.allclose(got["anisotropy"], an, rtol=1e-11, atol=1e-13, equal_nan=True)
+    scale = np.abs(np.asarray(p.mass_type)[t - 1][:, None] * st["v"]).sum()
+    assert np.all(np.abs(got["momentum"] - mom) <= 1e-13 * scale)
