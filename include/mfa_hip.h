@@ -38,7 +38,8 @@ extern "C" {
 typedef struct mfa_system_params {
     int32_t n_mon, n_chain;       /* grafted chains (init_system.f90:33) */
     int32_t n_mon_d, n_chain_d;   /* melt / solvent chains (init_system.f90:36) */
-    int32_t thermostat;           /* 1: DPD pair forces (THERMOSTAT==0, control_simulation.h:59) */
+    int32_t thermostat;           /* 1: DPD pair forces (THERMOSTAT==0, control_simulation.h:59);
+                                     2: Langevin bead forces (THERMOSTAT==1, lgv_forces.f90) */
     int32_t dpd_cut_off;          /* 1: DPD_CUT_OFF defined (control_simulation.h:63) */
     int32_t force_switch_on;      /* FORCE_SWITCH_ON (control_simulation.h:73) */
     int32_t bending;              /* BENDING (control_simulation.h:47) */

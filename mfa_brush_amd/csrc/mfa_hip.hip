@@ -262,7 +262,7 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
     // the thermostat acts beyond it (DPD_CUT_OFF), plus the skin
     for (int t = 0; t < 16; ++t) {
         double r = p.range_2[t] > 0.0 ? sqrt(p.range_2[t]) : 0.0;
-        if (p.thermostat && p.dpd_cut_off) r = std::max(r, sqrt(p.r_cut_dpd_2));
+        if (p.thermostat == 1 && p.dpd_cut_off) r = std::max(r, sqrt(p.r_cut_dpd_2));
         d.rl2[t] = r > 0.0 ? (r + p.skin) * (r + p.skin) : -1.0;
     }
     d.seed = p.seed;
@@ -357,7 +357,7 @@ static void enqueue_vv1(mfa_ctx *c) {
 static void enqueue_pair(mfa_ctx *c) {
     Timer t(c, K_PAIR);
     bool dpd = false;
-    for (auto &p : c->hp) dpd = dpd || p.thermostat;
+    for (auto &p : c->hp) dpd = dpd || p.thermostat == 1;
     const int nb = c->n_blocks;
     if (c->prec == MFA_PREC_MIXED) {
         if (dpd) {

@@ -198,6 +198,37 @@ __device__ __forceinline__ float pair_gauss(uint64_t seed, uint64_t t, int i, in
     return gauss_from_bits(c[0], c[1]);
 }
 
+// Langevin noise of bead i along axis d at time index t (lgv_forces.f90:14-20,
+// one normal per (i, d, t)).  The counter's second word carries d with the
+// top bit set, so it never equals a pair counter (second word: a bead index).
+__device__ __forceinline__ float lgv_gauss(uint64_t seed, uint64_t t, int i, int d) {
+    uint32_t c[4];
+    c[0] = (uint32_t)i;
+    c[1] = 0x80000000u | (uint32_t)d;
+    c[2] = (uint32_t)t;
+    c[3] = (uint32_t)(t >> 32);
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return gauss_from_bits(c[0], c[1]);
+}
+
+// Langevin force on bead i (local index) of type ti, lgv_forces.f90:14-25:
+// (sig g_d) sqrt(m) - (gamma m) v_d, added to f.
+__device__ __forceinline__ void lgv_bead(const SysDev &P, uint64_t t, int i, int ti, double4 v, double4 &f) {
+#pragma clang fp contract(off)
+    const double m = P.mass_type[(ti - 1) & 3];
+    const double sm = sqrt(m);
+    const double vv[3] = {v.x, v.y, v.z};
+    double add[3];
+    for (int d = 0; d < 3; ++d) {
+        double vd = P.sig * (double)lgv_gauss(P.seed, t, i, d) * sm;
+        vd = vd - P.friction * m * vv[d];
+        add[d] = vd;
+    }
+    f.x = f.x + add[0];
+    f.y = f.y + add[1];
+    f.z = f.z + add[2];
+}
+
 __device__ __forceinline__ double ipow(double x, int n) {
     double r = 1.0;
     while (n) { if (n & 1) r *= x; x *= x; n >>= 1; }

@@ -475,6 +475,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         }
         double4 f = assign ? make_double4(0.0, 0.0, 0.0, 0.0) : frc[k];
         f.x += a.fx; f.y += a.fy; f.z += a.fz;
+        if (P.thermostat == 2) lgv_bead(P, (uint64_t)t_idx, oi - P.part_base, ti, vel[k], f);
         frc[k] = f;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
@@ -753,6 +754,7 @@ __global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, 
         }
         double4 f = assign ? make_double4(0.0, 0.0, 0.0, 0.0) : frc[k];
         f.x += a.fx; f.y += a.fy; f.z += a.fz;
+        if (P.thermostat == 2) lgv_bead(P, X.t_idx, oi - P.part_base, ti, CUR(B.vel)[k], f);
         frc[k] = f;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;

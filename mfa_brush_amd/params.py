@@ -44,7 +44,10 @@ class Flags:
     brush_in_good_sv BRUSH_IN_GOOD_SV (:46), bending BENDING (:47),
     bending_melt BENDING_MELT (:48), thermostat THERMOSTAT==0 (:59),
     dpd_cut_off DPD_CUT_OFF (:63, None = undefined), force_switch_on
-    FORCE_SWITCH_ON (:73), poiseuille POISEUILLE (:80), l_box L_BOX (:105).
+    FORCE_SWITCH_ON (:73), poiseuille POISEUILLE (:80), l_box L_BOX (:105),
+    langevin THERMOSTAT==1 with the Langevin bead forces of lgv_forces.f90
+    (used when thermostat is off; the reference's current fluid_fluid.f90
+    leaves its call commented out, :455-458).
     WALL is fixed to 2 (implicit 9-3 walls) and SYMMETRY to 0 (channel).
     """
     solvent: int = 3
@@ -57,6 +60,7 @@ class Flags:
     force_switch_on: bool = True
     poiseuille: bool = True
     l_box: int = 1
+    langevin: bool = False
 
 
 @dataclass

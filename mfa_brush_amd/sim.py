@@ -25,7 +25,7 @@ def to_c(p: SimParams, t_idx0: int = 0) -> _lib.SystemParams:
     s = _lib.SystemParams()
     fl = p.flags
     s.n_mon, s.n_chain, s.n_mon_d, s.n_chain_d = p.n_mon, p.n_chain, p.n_mon_d, p.n_chain_d
-    s.thermostat = int(fl.thermostat)
+    s.thermostat = 1 if fl.thermostat else (2 if getattr(fl, "langevin", False) else 0)
     s.dpd_cut_off = int(fl.dpd_cut_off is not None)
     s.force_switch_on = int(fl.force_switch_on)
     s.bending, s.bending_melt, s.poiseuille = int(fl.bending), int(fl.bending_melt), int(fl.poiseuille)

@@ -112,6 +112,36 @@ float orc_pair_gauss(uint64_t seed, uint64_t step, int32_t i, int32_t j) {
     return orc_gauss_from_bits(out[0], out[1]);
 }
 
+/* Langevin noise of bead i along axis d at time index step: one normal per
+ * (i, d, step).  The counter's second word carries d with the top bit set,
+ * so it never equals a pair counter, whose second word is a bead index. */
+float orc_lgv_gauss(uint64_t seed, uint64_t step, int32_t i, int32_t d) {
+    uint32_t ctr[4], key[2], out[4];
+    ctr[0] = (uint32_t)i;
+    ctr[1] = 0x80000000u | (uint32_t)d;
+    ctr[2] = (uint32_t)step;
+    ctr[3] = (uint32_t)(step >> 32);
+    key[0] = (uint32_t)seed;
+    key[1] = (uint32_t)(seed >> 32);
+    orc_philox4x32_10(ctr, key, out);
+    return orc_gauss_from_bits(out[0], out[1]);
+}
+
+/* lgv_forces.f90:14-25 for bead i: (sig g_d) sqrt(m) - (gamma m) v_d added to
+ * its force.  The reference applies it per particle inside the pair loop
+ * (fluid_fluid_old.f90:233-236; commented out in fluid_fluid.f90:455-458). */
+static void lgv_bead(const orc_params *p, const double *v, const int *a_type, int i, uint64_t step, int noise,
+                     double *force) {
+    const double m = p->mass_type[(a_type[i] - 1) & 3];
+    const double sm = sqrt(m);
+    for (int d = 0; d < 3; ++d) {
+        const double g = noise ? (double)orc_lgv_gauss(p->seed, step, i, d) : 0.0;
+        double vd = p->sig * g * sm;
+        vd = vd - p->friction * m * v[3 * i + d];
+        force[3 * i + d] = force[3 * i + d] + vd;
+    }
+}
+
 /* ------------------------------------------------------------------------
  * Cell list (cell_list.f90)
  * ---------------------------------------------------------------------- */
@@ -328,13 +358,13 @@ static inline void pair_body(const orc_params *p, const double *r0, const double
         double fl[3] = {rdm * dr[0], rdm * dr[1], rdm * dr[2]};
         f_ipart[0] -= fl[0]; f_ipart[1] -= fl[1]; f_ipart[2] -= fl[2];
         force[3 * j + 0] += fl[0]; force[3 * j + 1] += fl[1]; force[3 * j + 2] += fl[2];
-        if (p->thermostat && !p->dpd_cut_off) {
+        if (p->thermostat == 1 && !p->dpd_cut_off) {
             double g = noise ? (double)orc_pair_gauss(p->seed, step, i, j) : 0.0;
             dpd_pair(p, dr, r_2, inv_sqrt_r_2, ti, tj, v + 3 * i, v + 3 * j, g,
                      force + 3 * i, force + 3 * j);
         }
     }
-    if (p->thermostat && p->dpd_cut_off) {
+    if (p->thermostat == 1 && p->dpd_cut_off) {
         if (ti == 1 || tj == 1) return;                 /* heads excluded, :223-236 */
         if (r_2 < p->r_cut_dpd_2) {
             /* the reference reuses the last 1/r when the LJ branch did not run
@@ -402,6 +432,8 @@ void orc_fluid_fluid(const orc_params *p, const double *r0, const double *v, con
             free(fl);
         }
         *v_fluid_fluid = v_ff;
+        if (p->thermostat == 2)
+            for (int i = 0; i < p->n_part; ++i) lgv_bead(p, v, a_type, i, step, noise, force);
         return;
     }
 #endif
@@ -409,6 +441,8 @@ void orc_fluid_fluid(const orc_params *p, const double *r0, const double *v, con
         ff_cell(p, r0, v, a_type, part_in_cell, r_nei, cell_neigh_ls, n_cells_tot, n_nei,
                 r_2_min, step, noise, c, force, &v_ff);
     *v_fluid_fluid = v_ff;
+    if (p->thermostat == 2)
+        for (int i = 0; i < p->n_part; ++i) lgv_bead(p, v, a_type, i, step, noise, force);
 }
 
 /* check_fluid_fluid, util.f90:284-330: O(N^2) brute-force LJ energy with the
@@ -517,6 +551,8 @@ void orc_fluid_fluid_brute(const orc_params *p, const double *r0, const double *
         force[3 * i + 2] += fi[2];
     }
     *v_ff = e;
+    if (p->thermostat == 2)
+        for (int i = 0; i < p->n_part; ++i) lgv_bead(p, v, a_type, i, step, noise, force);
 }
 
 /* ------------------------------------------------------------------------

@@ -27,7 +27,8 @@ typedef struct {
     int n_part;            /* n_mon_tot */
     int n_mon, n_chain;    /* grafted chains: beads 1..n_mon*n_chain, head first */
     int n_mon_d, n_chain_d;/* melt / solvent chains after the brush */
-    int thermostat;        /* 1: DPD pair forces on (reference THERMOSTAT==0) */
+    int thermostat;        /* 1: DPD pair forces (THERMOSTAT==0); 2: Langevin bead forces
+                              (THERMOSTAT==1, lgv_forces.f90) */
     int dpd_cut_off;       /* 1: DPD_CUT_OFF defined (own cutoff, heads excluded) */
     int force_switch_on;   /* FORCE_SWITCH_ON defined */
     int bending;           /* BENDING (brush chains) */
@@ -61,6 +62,7 @@ typedef struct {
 /* ---- RNG: Philox-4x32-10 (Salmon et al., SC'11) and the pair normal ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float orc_pair_gauss(uint64_t seed, uint64_t step, int32_t i, int32_t j);
+float orc_lgv_gauss(uint64_t seed, uint64_t step, int32_t i, int32_t d);
 
 /* ---- cell list (cell_list.f90) ---- */
 void orc_make_binning(int bin_case, const double boundary[3], double r_cut,

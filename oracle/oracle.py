@@ -110,7 +110,7 @@ def make_params(p) -> OrcParams:
     fl = p.flags
     o.n_part = p.n_part
     o.n_mon, o.n_chain, o.n_mon_d, o.n_chain_d = p.n_mon, p.n_chain, p.n_mon_d, p.n_chain_d
-    o.thermostat = int(fl.thermostat)
+    o.thermostat = 1 if fl.thermostat else (2 if getattr(fl, "langevin", False) else 0)
     o.dpd_cut_off = int(fl.dpd_cut_off is not None)
     o.force_switch_on = int(fl.force_switch_on)
     o.bending, o.bending_melt, o.poiseuille = int(fl.bending), int(fl.bending_melt), int(fl.poiseuille)

@@ -482,3 +482,34 @@ def test_chain_observables_vs_oracle(system):
     assert np.allclose(got["anisotropy"], an, rtol=1e-11, atol=1e-13, equal_nan=True)
     scale = np.abs(np.asarray(p.mass_type)[t - 1][:, None] * st["v"]).sum()
     assert np.all(np.abs(got["momentum"] - mom) <= 1e-13 * scale)
+
+
+def _langevin_c1():
+    p, t, r0, v = _perturbed_c1(thermostat=False, rc_dpd=None)
+    return p.replace(flags=Flags(**{**p.flags.__dict__, "langevin": True})), t, r0, v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_langevin_forces_vs_oracle(oracle_mod, precision):
+    """Langevin bead forces (lgv_forces.f90:14-25) with the same Philox
+    noise on the device and in the oracle: FP64 within the FP64 bar, mixed
+    within 1e-5 of the largest force."""
+    p, t, r0, v = _langevin_c1()
+    with Simulation(p, precision=precision, t_idx0=17) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+    fo, _ = _oracle(oracle_mod, p).fluid_fluid(r0, v, t, step=17, noise=1)
+    if precision == "fp64":
+        _close(f, fo)
+    else:
+        err = np.linalg.norm(f - fo, axis=1).max() / np.linalg.norm(fo, axis=1).max()
+        assert err < 1e-5, err
+
+
+def test_langevin_thermostat_holds_kt():
+    """The Langevin thermostat (friction gamma, sig = sqrt(2 kT gamma / dt))
+    holds kT = 1 within 5 % on the channel."""
+    p, t, r0, v = _langevin_c1()
+    kt = _mean_kt(p, t, r0, v, "mixed")
+    assert abs(kt - 1.0) < 0.05, kt
