@@ -38,6 +38,7 @@ static int fail(int code, const std::string &msg) {
 namespace {
 
 constexpr int GATED_GRID = 1024;     // grid of the rebuild kernels (grid-stride)
+constexpr int GRAPH_STEPS = 8;       // steps per replayed graph (plus a one-step graph for the rest)
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -83,8 +84,10 @@ struct mfa_ctx {
     double k_ms[K_NK] = {0, 0, 0, 0};
     long long k_n[K_NK] = {0, 0, 0, 0};
     // graph
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;      // one step
     hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec_k = nullptr;    // GRAPH_STEPS steps
+    hipGraph_t graph_k = nullptr;
 };
 
 static hipEvent_t take_event(mfa_ctx *c) {
@@ -188,6 +191,8 @@ static void free_all(mfa_ctx *c) {
     for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gexec_k) (void)hipGraphExecDestroy(c->gexec_k);
+    if (c->graph_k) (void)hipGraphDestroy(c->graph_k);
     if (c->st) (void)hipStreamDestroy(c->st);
 }
 
@@ -423,6 +428,21 @@ static int check_ctx(mfa_ctx *c, bool need_state = true) {
 static void invalidate_graph(mfa_ctx *c) {
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->gexec_k) { (void)hipGraphExecDestroy(c->gexec_k); c->gexec_k = nullptr; }
+    if (c->graph_k) { (void)hipGraphDestroy(c->graph_k); c->graph_k = nullptr; }
+}
+
+// Captures `steps` consecutive steps into one executable graph.  Every step
+// reads its time index and rebuild gates from device memory, so identical
+// captured steps advance the state exactly as separate launches do.
+static int capture_steps(mfa_ctx *c, int steps, hipGraph_t *g, hipGraphExec_t *ge) {
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < steps; ++i) enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->st, g);
+    if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("capture: ") + hipGetErrorString(e)); }
+    e = hipGraphInstantiate(ge, *g, nullptr, nullptr, 0);
+    if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("instantiate: ") + hipGetErrorString(e)); }
+    return 0;
 }
 
 // ===========================================================================
@@ -720,15 +740,15 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
         HIPCHK(hipGetLastError());
         return 0;
     }
-    if (!c->gexec) {
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c);
-        hipError_t e = hipStreamEndCapture(c->st, &c->graph);
-        if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("capture: ") + hipGetErrorString(e)); }
-        e = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) { invalidate_graph(c); return fail(MFA_E_HIP, std::string("instantiate: ") + hipGetErrorString(e)); }
+    int i = 0;
+    if (n_steps >= GRAPH_STEPS) {
+        if (!c->gexec_k)
+            if (int rc = capture_steps(c, GRAPH_STEPS, &c->graph_k, &c->gexec_k)) return rc;
+        for (; i + GRAPH_STEPS <= n_steps; i += GRAPH_STEPS) HIPCHK(hipGraphLaunch(c->gexec_k, c->st));
     }
-    for (int i = 0; i < n_steps; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    if (i < n_steps && !c->gexec)
+        if (int rc = capture_steps(c, 1, &c->graph, &c->gexec)) return rc;
+    for (; i < n_steps; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->st));
     return 0;
 }
 
