@@ -668,6 +668,7 @@ int mfa_hip_check_skin(mfa_ctx *c, int32_t *f_skin) {
 int mfa_hip_fluid_fluid(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     enqueue_rebuild(c);       // pending rebuild after set_state / verlet_positions (gated)
+    if (c->prec == MFA_PREC_MIXED) LAUNCH(k_velf_sync, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
     enqueue_pair(c);
     HIPCHK(hipGetLastError());
     return 0;

@@ -55,11 +55,22 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
         double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
         if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
         if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
-        float4 pf = CUR(B.posf)[k];
-        pf.x = (float)rx; pf.y = (float)ry; pf.z = (float)rz;
-        CUR(B.posf)[k] = pf;
+        CUR(B.posf)[k] = make_float4((float)rx, (float)ry, (float)rz, mkwf(worig(r.w), wtype(r.w)));
         CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
     }
+}
+
+// FP32 velocity copy refreshed from the FP64 velocities (mixed path): the
+// routine-level pair evaluation may follow any routine, not only k_vv1.
+__global__ __launch_bounds__(BLK) void k_velf_sync(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+                                                  Bufs B) {
+    const int s = blk_sys[blockIdx.x];
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k - P.slot_base >= P.n_part) return;
+    const double4 v = CUR(B.vel)[k];
+    CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,7 +275,9 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
             ke = P.mass_type[(ti - 1) & 3] * (v.x * v.x + v.y * v.y + v.z * v.z);
             CUR(B.vel)[k] = v;
             CUR(B.acc)[k] = a;
-            if (MIXED) CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+            // (the fused step's next pair evaluation reads the FP32 copy that
+            // k_vv1 writes after its half kick)
+            if (MIXED && !FUSED) CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
         f4.x = F.x; f4.y = F.y; f4.z = F.z;
         B.frc[k] = f4;
