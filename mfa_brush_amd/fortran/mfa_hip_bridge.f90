@@ -174,7 +174,11 @@ contains
         p%n_mon = n_mon; p%n_chain = n_chain; p%n_mon_d = n_mon_d; p%n_chain_d = n_chain_d
 #if THERMOSTAT == 0
         p%thermostat = 1
+#elif defined(MFA_HIP_LANGEVIN)
+        ! lgv_forces.f90, per bead, as fluid_fluid_old.f90:233-236 calls it
+        p%thermostat = 2
 #else
+        ! fluid_fluid.f90:455-458 leaves the lgv_forces call commented out
         p%thermostat = 0
 #endif
 #ifdef DPD_CUT_OFF
