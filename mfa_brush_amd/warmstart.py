@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 import sys
 import time
 from dataclasses import dataclass, asdict
@@ -203,21 +204,34 @@ def main(argv=None):
     """Train on batched runs, then compare plain and warm starts on held-out
     channels.  Arguments: report path, training channels, equilibration and
     averaging steps of the training runs, blocks of 1000 steps per held-out
-    run, held-out box side (sigma).  Writes a JSON report."""
+    run, held-out box side (sigma), optional training-profile file (.npz:
+    loaded if present, else written after the training runs), optional
+    comma-separated held-out indices (0-2, default all).  Writes a JSON
+    report."""
     argv = sys.argv[1:] if argv is None else argv
     out_path = argv[0] if argv else "warmstart_report.json"
     n_train = int(argv[1]) if len(argv) > 1 else 48
     equil, avg = (int(argv[2]), int(argv[3])) if len(argv) > 3 else (40000, 20000)
     n_blocks = int(argv[4]) if len(argv) > 4 else 100
     side = float(argv[5]) if len(argv) > 5 else 56.0
+    prof_path = argv[6] if len(argv) > 6 and argv[6] != "-" else None
+    which = [int(x) for x in argv[7].split(",") if x] if len(argv) > 7 else [0, 1, 2]
     rep = {"n_train": n_train, "equil_steps": equil, "avg_steps": avg, "n_bins": N_BINS, "lz": LZ}
     t0 = time.perf_counter()
     specs = training_specs(n_train)
-    profs, fl, nb = equilibrium_profiles(specs, equil, avg)
-    rep["train_sim_seconds"] = time.perf_counter() - t0
-    rep["train_beads"] = nb
-    rep["train_flags"] = fl
-    print("training profiles: %.1f s" % rep["train_sim_seconds"], flush=True)
+    if prof_path and os.path.exists(prof_path):
+        with np.load(prof_path) as z:
+            profs = list(z["profiles"])
+            assert len(profs) == n_train and int(z["equil"]) == equil and int(z["avg"]) == avg
+        rep["train_profiles_from"] = prof_path
+    else:
+        profs, fl, nb = equilibrium_profiles(specs, equil, avg)
+        rep["train_sim_seconds"] = time.perf_counter() - t0
+        rep["train_beads"] = nb
+        rep["train_flags"] = fl
+        print("training profiles: %.1f s" % rep["train_sim_seconds"], flush=True)
+        if prof_path:
+            np.savez(prof_path, profiles=np.array(profs), equil=equil, avg=avg)
     t0 = time.perf_counter()
     mlp, hist = train(specs, profs)
     rep["train_mlp_seconds"] = time.perf_counter() - t0
@@ -225,7 +239,7 @@ def main(argv=None):
     held = [Spec(0.30, 32, 1.0, side=side, seed=901), Spec(0.12, 16, 0.8, side=side, seed=902),
             Spec(0.45, 32, 1.3, side=side, seed=903)]
     rep["held_out"] = []
-    for s in held:
+    for s in [held[i] for i in which]:
         plain, n_part, secs_p, flp = block_profiles(s, None, n_blocks)
         warm, _, secs_w, flw = block_profiles(s, mlp, n_blocks)
         late_p, late_w = plain[n_blocks // 2:].mean(0), warm[n_blocks // 2:].mean(0)
