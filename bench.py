@@ -161,8 +161,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=40,
-                    help="steps of the CPU baseline sample (about 10 s of CPU work at 1M beads)")
+    ap.add_argument("--cpu-steps", type=int, default=80,
+                    help="steps of the CPU baseline sample (10-20 s of CPU work at 1M beads on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-list", action="store_true", help="pair kernels scan the cells every step")

@@ -596,8 +596,12 @@ __device__ __forceinline__ void pair32_dpd(const SysDev &P, const float *tf, con
     }
 }
 
+// Six waves per SIMD: caps the kernel at 80 VGPRs (91 without the hint, five
+// waves) for a 12-byte spill; measured 5-9 % faster on the pair step at 0.3-5M
+// beads.  Seven waves (72 VGPRs, 40-byte spill) is 60 % slower, and five waves
+// forced on k_pair64 (96 VGPRs) is slower than its natural four.
 template <bool DPD, bool VIR>
-__global__ __launch_bounds__(BLK) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
                                                double *__restrict__ partial, int assign) {
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
