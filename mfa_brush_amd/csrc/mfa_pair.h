@@ -58,6 +58,14 @@ constexpr int DPD_UNROLL = 2;     // deferred pairs whose partner loads are issu
 __device__ __forceinline__ size_t nl_index(int k, int q, size_t stride) {
     return ((size_t)(q >> 2) * stride + (size_t)k) * 4 + (size_t)(q & 3);
 }
+// Slot i of a per-slot array through an unsigned 32-bit byte offset, so the
+// load takes the uniform base in scalar registers and one 32-bit lane offset
+// (no 64-bit address add per gather).  Lists hold slots below 2^24 (the host
+// refuses more), so i * 32 stays below 2^32.
+template <typename T>
+__device__ __forceinline__ T ld_slot(const T *__restrict__ base, unsigned i) {
+    return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+}
 __device__ __forceinline__ int nl_code(int dx, int dy, int dz, int S) {
     const int w = 2 * S + 1;
     return (dx + S) + w * ((dy + S) + w * (dz + S));
@@ -411,17 +419,19 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
+            const unsigned nsl = (unsigned)B.n_slots;
             for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
-                int mm[NL_UNROLL];
+                unsigned mm[NL_UNROLL];
                 double4 pj[NL_UNROLL];
                 const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
-                mm[0] = e4.x >> NL_CODE_BITS; mm[1] = e4.y >> NL_CODE_BITS;
-                mm[2] = e4.z >> NL_CODE_BITS; mm[3] = e4.w >> NL_CODE_BITS;
+                // unsigned slots: one compare is the range check (as in k_pair32)
+                mm[0] = (unsigned)e4.x >> NL_CODE_BITS; mm[1] = (unsigned)e4.y >> NL_CODE_BITS;
+                mm[2] = (unsigned)e4.z >> NL_CODE_BITS; mm[3] = (unsigned)e4.w >> NL_CODE_BITS;
 #pragma unroll
-                for (int u = 0; u < NL_UNROLL; ++u) pj[u] = pos[in_range(mm[u], B.n_slots) ? mm[u] : k];
+                for (int u = 0; u < NL_UNROLL; ++u) pj[u] = ld_slot(pos, min(mm[u], nsl - 1u));
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    if (q0 + u < nn && guard(B, in_range(mm[u], B.n_slots), G_SLOT, k)) visit(mm[u], pj[u]);
+                    if (q0 + u < nn && guard(B, mm[u] < nsl, G_SLOT, k)) visit((int)mm[u], pj[u]);
                 }
             }
         } else {
@@ -686,21 +696,26 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
+            const unsigned nsl = (unsigned)B.n_slots;
             for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
                 int ent[NL_UNROLL];
                 float4 pj[NL_UNROLL];
                 const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
                 ent[0] = e4.x; ent[1] = e4.y; ent[2] = e4.z; ent[3] = e4.w;
+                // a negative entry shifts to a slot of at least 2^24, above
+                // every valid slot, so one unsigned compare is the whole range
+                // check; the speculative gather is clamped and an
+                // out-of-range slot is never visited
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    const int m = ent[u] >> NL_CODE_BITS;
-                    pj[u] = posf[in_range(m, B.n_slots) ? m : k];
+                    const unsigned m = (unsigned)ent[u] >> NL_CODE_BITS;
+                    pj[u] = ld_slot(posf, min(m, nsl - 1u));
                 }
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    const int m = ent[u] >> NL_CODE_BITS;
-                    if (q0 + u < nn && guard(B, in_range(m, B.n_slots), G_SLOT, k))
-                        visit(m, ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
+                    const unsigned m = (unsigned)ent[u] >> NL_CODE_BITS;
+                    if (q0 + u < nn && guard(B, m < nsl, G_SLOT, k))
+                        visit((int)m, ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
                 }
             }
         } else {
