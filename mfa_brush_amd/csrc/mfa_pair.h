@@ -310,43 +310,61 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                 const float qfy = pfi.y - (float)dy * (float)P.l_cell[1];
                 const float qfz = pfi.z - (float)dz * (float)P.l_cell[2];
                 const double qx = pi.x - sx, qy = pi.y - sy;
-                // partners from the stage when the cell lies inside it, else
-                // from memory (padded buffer); four loads in flight, loads past
-                // the cell's end are not used
-                auto scan = [&](auto load) {
-                    for (int m0 = rg.x; m0 < rg.y; m0 += NL_UNROLL) {
-                        bool take[NL_UNROLL];
-                        Bead pj[NL_UNROLL];
+                // One group of four candidates from m0.  TAIL: the group may
+                // run past the cell's end (only the cell's last group does);
+                // SELF: the cell is the bead's own, the only one holding slot
+                // k (boxes have at least 2S+1 cells per periodic axis), so
+                // both tests leave the other groups' per-candidate work.
+                auto group = [&](auto load, int m0, auto tail, auto self) {
+                    bool take[NL_UNROLL];
+                    Bead pj[NL_UNROLL];
 #pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) pj[u] = load(m0 + u);
+                    for (int u = 0; u < NL_UNROLL; ++u) pj[u] = load(m0 + u);
 #pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) {
-                            if constexpr (MIXED) {
-                                const float ex = qfx - pj[u].x, ey = qfy - pj[u].y, ez = qfz - pj[u].z;
-                                take[u] = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <
-                                          rl2f[trow + (int)(__float_as_uint(pj[u].w) & 7u)];
-                            } else {
-                                const double ex = qx - pj[u].x, ey = qy - pj[u].y, ez = pi.z - pj[u].z;
-                                take[u] = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) <
-                                          rl2d[trow + wtype(pj[u].w)];
-                            }
+                    for (int u = 0; u < NL_UNROLL; ++u) {
+                        if constexpr (MIXED) {
+                            const float ex = qfx - pj[u].x, ey = qfy - pj[u].y, ez = qfz - pj[u].z;
+                            take[u] = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <
+                                      rl2f[trow + (int)(__float_as_uint(pj[u].w) & 7u)];
+                        } else {
+                            const double ex = qx - pj[u].x, ey = qy - pj[u].y, ez = pi.z - pj[u].z;
+                            take[u] = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex)) <
+                                      rl2d[trow + wtype(pj[u].w)];
                         }
+                    }
 #pragma unroll
-                        for (int u = 0; u < NL_UNROLL; ++u) {
-                            const int m = m0 + u;
-                            if (m < rg.y && m != k && take[u]) {
-                                b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
-                                if ((n & 3) == 3) {
-                                    if (n < cap) *gp = make_int4(b0, b1, b2, b3);
-                                    gp += group_step;
-                                }
-                                ++n;
+                    for (int u = 0; u < NL_UNROLL; ++u) {
+                        const int m = m0 + u;
+                        bool ok = take[u];
+                        if constexpr (decltype(tail)::value) ok = ok && m < rg.y;
+                        if constexpr (decltype(self)::value) ok = ok && m != k;
+                        if (ok) {
+                            b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
+                            if ((n & 3) == 3) {
+                                if (n < cap) *gp = make_int4(b0, b1, b2, b3);
+                                gp += group_step;
                             }
+                            ++n;
                         }
                     }
                 };
-                if (rg.x >= lo && rg.y <= lo + cnt) scan([&](int m) { return stage[m - lo]; });
-                else scan([&](int m) { return src[m]; });
+                // partners from the stage when the cell lies inside it, else
+                // from memory (padded buffer); four loads in flight, loads past
+                // the cell's end are not used
+                auto scan = [&](auto load, auto self) {
+                    int m0 = rg.x;
+                    for (; m0 + NL_UNROLL <= rg.y; m0 += NL_UNROLL)
+                        group(load, m0, std::false_type{}, self);
+                    if (m0 < rg.y) group(load, m0, std::true_type{}, self);
+                };
+                const bool own = dx == 0 && dy == 0 && dz == 0;
+                if (rg.x >= lo && rg.y <= lo + cnt) {
+                    auto ld = [&](int m) { return stage[m - lo]; };
+                    if (own) scan(ld, std::true_type{}); else scan(ld, std::false_type{});
+                } else {
+                    auto ld = [&](int m) { return src[m]; };
+                    if (own) scan(ld, std::true_type{}); else scan(ld, std::false_type{});
+                }
             }
         }
     }
