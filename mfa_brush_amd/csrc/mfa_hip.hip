@@ -37,7 +37,10 @@ static int fail(int code, const std::string &msg) {
 
 namespace {
 
-constexpr int GATED_GRID = 1024;     // grid of the rebuild kernels (grid-stride)
+// grid of the rebuild kernels (grid-stride): eight blocks per CU keep more
+// tiles' loads and atomics in flight than four (rebuild 566 -> 541 us per
+// step on 256 replicas, 3.5M beads) and cost nothing measurable when idle
+constexpr int GATED_GRID = 2048;
 constexpr int GRAPH_STEPS = 8;       // steps per replayed graph (plus a one-step graph for the rest)
 
 struct TimedLaunch {
