@@ -392,6 +392,31 @@ def test_full_c2_pair_count_and_momentum(oracle_mod):
     assert sums["mixed"][0] < 1e-11 * sums["mixed"][1] * n, sums
 
 
+def test_full_c2_fused_steps_vs_oracle(oracle_mod):
+    """Full C2 (998,784 beads): 16 fused FP64 steps, graph-replayed, against
+    the oracle's linked-cell steps.  At this size the grid-stride rebuild
+    kernels give blocks one or two tiles each, and the skin test triggers
+    device-gated rebuilds after the initial one."""
+    p, t, r0, v = configs.channel_c2(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    a0 = np.zeros_like(r0)
+    n = 16
+    ro, vo, ao, fo, eo, fho = oracle_mod.Oracle(p, omp=True).md_steps(r0, v, a0, t, 0, n, noise=1)
+    with Simulation(p) as s:
+        s.set_state(r0, v, t, a0)
+        s.md_steps(n, graph=True)
+        st = s.get_state()
+        e = s.energies()
+        rebuilds = s.flags()["rebuilds"]
+    print("C2 rebuilds in %d steps (with the initial one): %d" % (n, rebuilds))
+    assert rebuilds >= 2
+    _close(st["r0"], ro, 1e-9)
+    _close(st["v"], vo, 1e-6)
+    _close(st["force"], fo, 1e-6)
+    for k in ("v_fluid_fluid", "v_intra_molec", "v_fluid_wall", "t_fluid"):
+        assert e[k] == pytest.approx(eo[k], rel=1e-6, abs=1e-9), k
+
+
 def test_profiles_vs_oracle(oracle_mod):
     p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
     with Simulation(p) as s:
