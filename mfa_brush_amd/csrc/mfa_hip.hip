@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -183,7 +184,7 @@ static int to_device(mfa_ctx *c, void *dst, const void *src, size_t bytes, const
 static void free_all(mfa_ctx *c) {
     auto F = [](void *p) { if (p) (void)hipFree(p); };
     F(c->B.pos); F(c->B.vel); F(c->B.acc); F(c->B.posb); F(c->B.posf); F(c->B.velf); F(c->B.cell_of);
-    F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.chan_rb); F(c->B.flags); F(c->B.t_idx);
+    F(c->B.frc); F(c->B.where); F(c->B.parity); F(c->B.rebuild); F(c->B.chan_rb); F(c->B.flags); F(c->B.t_idx); F(c->B.t_end); F(c->B.store_frc);
     F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
@@ -532,7 +533,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     }
     ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, n_sys); ALLOC(c->B.rebuild, 1);
     ALLOC(c->B.chan_rb, n_sys);
-    ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1);
+    ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1); ALLOC(c->B.t_end, 1); ALLOC(c->B.store_frc, 1);
     ALLOC(c->d_sys, n_sys); ALLOC(c->d_blk_sys, c->n_blocks); ALLOC(c->d_cell_sys, c->C_tot);
     ALLOC(c->d_cell_count, c->C_tot); ALLOC(c->d_pre, c->C_tot);
     ALLOC(c->d_bsum, (c->C_tot + 4 * BLK - 1) / (4 * BLK) + 1);
@@ -553,7 +554,12 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     // anything is enqueued on the context's non-blocking stream
     if (hipDeviceSynchronize() != hipSuccess) { free_all(c); delete c; return fail(MFA_E_HIP, "allocation sync failed"); }
     long long t0 = sys[0].t_idx0;
+    // until an md_steps call sets t_end, every fused step stores the force
+    const long long t_end0 = LLONG_MIN;
+    const int store1 = 1;
     if ((rc = to_device(c, c->d_sys, c->hs.data(), n_sys * sizeof(SysDev), "channel parameters")) ||
+        (rc = to_device(c, c->B.t_end, &t_end0, sizeof(long long), "last-step index")) ||
+        (rc = to_device(c, c->B.store_frc, &store1, sizeof(int), "force-store flag")) ||
         (rc = to_device(c, c->d_blk_sys, blk_sys.data(), c->n_blocks * sizeof(int), "block table")) ||
         (rc = to_device(c, c->d_cell_sys, cell_sys.data(), c->C_tot * sizeof(int), "cell table")) ||
         (rc = to_device(c, c->B.t_idx, &t0, sizeof(long long), "time index"))) {
@@ -739,6 +745,7 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
     if (int rc = check_ctx(c)) return rc;
     if (n_steps < 0) return fail(MFA_E_ARG, "md_steps: negative step count");
     ensure_cells(c);
+    if (n_steps > 0) hipLaunchKernelGGL(k_set_t_end, dim3(1), dim3(1), 0, c->st, c->B, (int)n_steps);
     if (!use_graph || c->timing) {
         for (int i = 0; i < n_steps; ++i) enqueue_step(c);
         HIPCHK(hipGetLastError());

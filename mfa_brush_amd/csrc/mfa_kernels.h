@@ -65,6 +65,8 @@ struct Bufs {
     int *chan_rb;       // per channel: this channel's cells are rebuilt this step
     int *flags;         // [0..3] anomaly counters, [4] list overflows, [6] offending index, [7] first guard code
     long long *t_idx;
+    long long *t_end;   // md_steps: time index of the call's last step
+    int *store_frc;     // fused step: k_bonded stores the total force (set by k_vv1)
     int *nl;            // Verlet list: entry q of slot k at nl[((q / 4) * n_slots + k) * 4 + q % 4]
     int *nl_n;          // list length per slot (> nl_cap: overflowed, the cells are scanned)
     int nl_cap;         // entries per slot; 0: no list, the pair kernels scan the cells

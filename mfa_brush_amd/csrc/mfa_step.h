@@ -19,6 +19,10 @@ __global__ __launch_bounds__(BLK) void k_vv1(const SysDev *__restrict__ sys, con
     const SysDev &P = sys[s];
     const int k = blockIdx.x * BLK + threadIdx.x;
     if (threadIdx.x == 0 && P.skin_mode == 0) { *B.rebuild = 1; B.chan_rb[s] = 1; }
+    // the fused step's k_bonded stores the total force only on the last step
+    // of an md_steps call (the next pair kernel overwrites it); t_idx is
+    // read here because k_bonded itself advances it
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.store_frc = *B.t_idx >= *B.t_end ? 1 : 0;
     if (k - P.slot_base >= P.n_part) return;
     double4 r = CUR(B.pos)[k];
     double4 v = CUR(B.vel)[k];
@@ -191,6 +195,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
     constexpr bool FUSED = (MODE & M_FUSED) != 0;
+    const bool store_f = !FUSED || *B.store_frc != 0;
     double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
     if (active) {
         const double4 ri = CUR(B.pos)[k];
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
             if (MIXED && !FUSED) CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
         f4.x = F.x; f4.y = F.y; f4.z = F.z;
-        B.frc[k] = f4;
+        if (store_f) B.frc[k] = f4;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
     if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
@@ -293,6 +298,10 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         if (blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
     }
 }
+
+// Time index of the last of the n steps an md_steps call enqueues (the
+// current index plus n - 1); read by k_vv1 at every fused step.
+__global__ void k_set_t_end(Bufs B, int n) { *B.t_end = *B.t_idx + n - 1; }
 
 // Per-channel reduction of the block partials in fixed order.
 __global__ __launch_bounds__(BLK) void k_reduce(const SysDev *__restrict__ sys, const double *__restrict__ partial,
