@@ -400,7 +400,10 @@ static void enqueue_step(mfa_ctx *c) {
     enqueue_vv1(c);
     enqueue_rebuild(c);
     enqueue_pair(c);
-    enqueue_bonded<M_FUSED>(c);
+    bool bend = false;
+    for (const SysDev &d : c->hs) bend = bend || d.bending || d.bending_melt;
+    if (bend) enqueue_bonded<M_FUSED>(c);
+    else enqueue_bonded<M_FUSED | M_NOBEND>(c);
 }
 
 // Reads the device guard word (one sync): a nonzero code means a kernel met

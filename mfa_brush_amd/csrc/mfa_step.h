@@ -144,7 +144,10 @@ __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm
     return o;
 }
 
-enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64 };
+// M_NOBEND: a fused step of channels that all have bending and melt bending
+// off; the bending code is compiled out (its registers cap the occupancy of
+// the fused kernel), its energy partials are still written (zero).
+enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64, M_NOBEND = 128 };
 
 __device__ __forceinline__ double4 bead_at(const Bufs &B, size_t cur_off, int gorig) {
     if (!guard(B, in_range(gorig, B.n_beads), G_ORIG, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
@@ -238,10 +241,11 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
             }
             if (str) atomicAdd(&B.flags[2], 1);
         }
-        if ((MODE & M_BEND) || (FUSED && P.bending)) {
+        constexpr bool FBEND = FUSED && !(MODE & M_NOBEND);
+        if ((MODE & M_BEND) || (FBEND && P.bending)) {
             if (brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bend);
         }
-        if ((MODE & M_BENDM) || (FUSED && P.bending_melt)) {
+        if ((MODE & M_BENDM) || (FBEND && P.bending_melt)) {
             if (!brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bendm);
         }
         if ((MODE & M_CONST) || (FUSED && P.poiseuille)) {
