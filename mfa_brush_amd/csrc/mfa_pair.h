@@ -120,6 +120,10 @@ __device__ __forceinline__ Pair64Ctx pair64_ctx(const SysDev &P, long long t_idx
     C.rdpd2 = P.r_cut_dpd_2; C.sig = P.sig; C.fric = P.friction; C.r_2_min = r_2_min;
     C.seed = P.seed; C.t_idx = (uint64_t)t_idx;
     C.part_base = P.part_base; C.dpd_cut = P.dpd_cut_off; C.fso = P.force_switch_on;
+    // a batch may mix thermostats: a channel whose thermostat is not DPD takes
+    // the DPD_CUT_OFF rule with an empty range, so no pair is flagged for a
+    // DPD part (no test per candidate)
+    if (P.thermostat != 1) { C.dpd_cut = 1; C.rdpd2 = -1.0; }
     return C;
 }
 
@@ -683,6 +687,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     X.seed = P.seed; X.part_base = P.part_base; X.dpd_cut = P.dpd_cut_off; X.fso = P.force_switch_on;
     X.rdpd2 = (float)P.r_cut_dpd_2;
     X.inv_rdpd = (float)(1.0 / sqrt(P.r_cut_dpd_2 > 0.0 ? P.r_cut_dpd_2 : 1.0));
+    // a channel whose thermostat is not DPD flags no pair for a DPD part (as
+    // in pair64_ctx: the DPD_CUT_OFF rule with an empty range)
+    if (P.thermostat != 1) { X.dpd_cut = 1; X.rdpd2 = -1.0f; }
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
