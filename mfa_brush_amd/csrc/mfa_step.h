@@ -201,6 +201,10 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     const bool store_f = !FUSED || *B.store_frc != 0;
     double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
     if (active) {
+        // no contraction: the kernel's variants (with and without the
+        // bending code) then round the walls, sums and kinetic energy alike,
+        // as the oracle does
+#pragma clang fp contract(off)
         const double4 ri = CUR(B.pos)[k];
         const int ti = wtype(ri.w);
         const int g = worig(ri.w);

@@ -297,18 +297,22 @@ def test_batched_replicas_equal_single_runs():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_bending_and_plain_channels_batched_equal_single_runs(precision):
-    """A batch with a bending channel runs the fused bonded kernel with the
-    bending code; a plain channel alone runs the variant compiled without it.
-    Both channels give the same bits batched and alone, forces included."""
+    """A batch of a DPD-thermostatted brush channel, a bending channel without
+    a thermostat and a Langevin channel.  The batch runs the DPD pair kernel
+    and the fused bonded kernel with the bending code; alone, the plain
+    channel runs the variant without it and the others the kernel without
+    DPD.  Every channel gives the same bits batched and alone, forces
+    included."""
     p0, t0, r00, v0 = configs.channel_scaled(7, n_mon=8, lz=14.0, seed=200, skin=0.3)
     p1, t1, r01, v1 = _chain_system(bending_melt=True)
-    ps, states = [p0, p1], [(t0, r00, v0), (t1, r01, v1)]
+    p2, t2, r02, v2 = _langevin_c1()
+    ps, states = [p0, p1, p2], [(t0, r00, v0), (t1, r01, v1), (t2, r02, v2)]
     with Simulation(ps, precision=precision) as sb:
         for i, (t, r0, v) in enumerate(states):
             sb.set_state(r0, v, t, isys=i)
         sb.md_steps(12, graph=True)
-        batched = [sb.get_state(i) for i in range(2)]
-        eb = [sb.energies(i) for i in range(2)]
+        batched = [sb.get_state(i) for i in range(3)]
+        eb = [sb.energies(i) for i in range(3)]
     for i, (t, r0, v) in enumerate(states):
         with Simulation(ps[i], precision=precision) as s:
             s.set_state(r0, v, t)
@@ -317,8 +321,8 @@ def test_bending_and_plain_channels_batched_equal_single_runs(precision):
             es = s.energies()
         for k in ("r0", "v", "a", "force"):
             assert np.array_equal(batched[i][k], single[k]), (i, k)
-        for k in ("v_fluid_fluid", "v_intra_molec", "v_bend", "v_bend_melt", "t_fluid"):
-            assert eb[i][k] == es[k], (i, k)
+        for k in ("v_fluid_fluid", "v_fluid_wall", "v_intra_molec", "v_bend", "v_bend_melt", "t_fluid"):
+            assert eb[i][k] == es[k], (i, k, eb[i][k], es[k])
 
 
 def test_mixed_precision_force_error(oracle_mod):
