@@ -295,6 +295,31 @@ def test_batched_replicas_equal_single_runs():
         assert eb[i]["v_fluid_fluid"] == es["v_fluid_fluid"]
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_md_steps_calls_compose(graph):
+    """md_steps(5) then md_steps(3) gives the bits of md_steps(8), forces
+    included: the fused step writes the total force back on the last step of
+    each call only, and the state read between the calls is that of step 5."""
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(5, graph=graph)
+        mid = s.get_state()
+        s.md_steps(3, graph=graph)
+        two = s.get_state()
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(5, graph=graph)
+        five = s.get_state()
+    with Simulation(p) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(8, graph=graph)
+        eight = s.get_state()
+    for k in ("r0", "v", "a", "force"):
+        assert np.array_equal(mid[k], five[k]), ("5", k)
+        assert np.array_equal(two[k], eight[k]), ("5+3", k)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_bending_and_plain_channels_batched_equal_single_runs(precision):
     """A batch of a DPD-thermostatted brush channel, a bending channel without
