@@ -127,7 +127,9 @@ int  mfa_hip_verlet_velocities(mfa_ctx *ctx);      /* verlet_velocities.f90:18-1
 
 /* Fused steps: n_steps of md_main.f90:77-185 in 4 kernels per step
  * (integrate+skin check, [gated cell rebuild], pair, bonded+wall+kick).
- * use_graph=1 captures one step in a hipGraph and replays it. */
+ * use_graph=1 replays captured graphs of 8 steps and of 1 step.  The force
+ * that mfa_hip_get_state returns afterwards is the total force of the last
+ * step (the steps before it do not write it back). */
 int  mfa_hip_md_steps(mfa_ctx *ctx, int32_t n_steps, int32_t use_graph);
 int  mfa_hip_synchronize(mfa_ctx *ctx);
 
