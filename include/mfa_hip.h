@@ -81,7 +81,11 @@ typedef struct mfa_energies {
     double v_bend;          /* bending.f90:100 */
     double v_bend_melt;     /* bending_melt.f90 */
     double t_fluid;         /* 0.5 sum m v^2, verlet_velocities.f90:132-153 */
-    double virial[6];       /* pair virial W = sum r_ij (x) F_ij, xx yy zz xy xz yz (no reference counterpart) */
+    double virial[6];       /* pair virial W = sum over interacting pairs of r_ij (x) F_ij, xx yy zz xy xz yz,
+                               r_ij the minimum-image separation and F_ij the whole pair force: LJ, DPD
+                               dissipative and DPD random parts (the reference's bulk-mode press_tensor,
+                               dpd_forces.f90:118-127, keeps only the dissipative DPD part; it has none in
+                               channel mode).  Zero while mfa_hip_set_virial is off */
 } mfa_energies;
 
 typedef struct mfa_flags {
@@ -131,6 +135,9 @@ int  mfa_hip_verlet_velocities(mfa_ctx *ctx);      /* verlet_velocities.f90:18-1
  * that mfa_hip_get_state returns afterwards is the total force of the last
  * step (the steps before it do not write it back). */
 int  mfa_hip_md_steps(mfa_ctx *ctx, int32_t n_steps, int32_t use_graph);
+/* Captures the graphs md_steps(use_graph=1) replays in the current timing
+ * mode, without running a step (a timed region then pays no capture). */
+int  mfa_hip_prepare_graphs(mfa_ctx *ctx);
 int  mfa_hip_synchronize(mfa_ctx *ctx);
 
 /* ---- observables and checks ------------------------------------------- */
@@ -158,11 +165,16 @@ int  mfa_hip_get_cells(mfa_ctx *ctx, int32_t isys, int32_t *n_cells, int32_t *ce
                        int32_t *cell_count, int32_t *cell_beads);
 
 /* ---- timing ------------------------------------------------------------
- * When enabled, each fused-step kernel launch is bracketed by hipEvents on
- * the context's stream; totals are in milliseconds. */
+ * Mode 1: each fused-step kernel group (vv1, rebuild, pair, bonded) is
+ * bracketed by hipEvents on the context's stream, and md_steps launches
+ * eagerly.  Mode 2: only the pair kernel is bracketed, and md_steps keeps
+ * replaying graphs: a ring of captured graphs carries the event records as
+ * graph nodes, read after every turn of the ring.  Mode 0: off.  Totals
+ * are in milliseconds. */
 int  mfa_hip_set_timing(mfa_ctx *ctx, int32_t on);
 int  mfa_hip_get_kernel_time(mfa_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
-int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off) */
+int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off); waits
+                                                        for the context's stream and drops the captured graphs */
 
 /* Verlet-list capacity (entries per bead) of the pair kernels: -1 automatic
  * (default), 0 no list (the pair kernels scan the 27 or 125 neighbour cells

@@ -72,6 +72,7 @@ SIGNATURES = {
     "mfa_hip_constant_force": [_vp, _i32],
     "mfa_hip_verlet_velocities": [_vp],
     "mfa_hip_md_steps": [_vp, _i32, _i32],
+    "mfa_hip_prepare_graphs": [_vp],
     "mfa_hip_synchronize": [_vp],
     "mfa_hip_get_energies": [_vp, _i32, _vp],
     "mfa_hip_get_f_on_heads": [_vp, _i32, _vp],

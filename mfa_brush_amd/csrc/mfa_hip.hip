@@ -43,6 +43,7 @@ namespace {
 // step on 256 replicas, 3.5M beads) and cost nothing measurable when idle
 constexpr int GATED_GRID = 2048;
 constexpr int GRAPH_STEPS = 8;       // steps per replayed graph (plus a one-step graph for the rest)
+constexpr int TIMING_RING = 4;       // timing graphs replayed in turn between two reads of their events
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -81,10 +82,14 @@ struct mfa_ctx {
     std::vector<char> uploaded;
     bool virial = false;
     int nl_request = -1;          // Verlet-list capacity: -1 automatic, 0 off, > 0 entries per bead
-    // timing
-    bool timing = false;
+    // timing: timing_mask has bit k set when kernel group k is bracketed by
+    // events; sink (non-null while a timing graph is captured) receives the
+    // captured event pairs instead of `pending`
+    int timing_mask = 0;
+    bool timing_graphs = false;          // pair-only mode: graph replay with event nodes
     std::vector<hipEvent_t> ev_pool;
     std::vector<TimedLaunch> pending;
+    std::vector<TimedLaunch> *sink = nullptr;
     double k_ms[K_NK] = {0, 0, 0, 0};
     long long k_n[K_NK] = {0, 0, 0, 0};
     // graph
@@ -92,6 +97,13 @@ struct mfa_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec_k = nullptr;    // GRAPH_STEPS steps
     hipGraph_t graph_k = nullptr;
+    // timing graphs: TIMING_RING captured graphs of GRAPH_STEPS steps, each
+    // with its own event pairs around the pair kernel; replays cycle through
+    // them and the events are read after every TIMING_RING replays, before
+    // any graph is replayed (and its events recorded) again
+    struct TGraph { hipGraph_t g = nullptr; hipGraphExec_t ge = nullptr; std::vector<TimedLaunch> ev; };
+    std::vector<TGraph> tring;
+    int tring_next = 0, tring_unread = 0;
 };
 
 static hipEvent_t take_event(mfa_ctx *c) {
@@ -110,29 +122,43 @@ struct Timer {
     int kid;
     hipEvent_t a = nullptr;
     Timer(mfa_ctx *c_, int k) : c(c_), kid(k) {
-        if (c->timing) { a = take_event(c); if (a) (void)hipEventRecord(a, c->st); }
+        if (c->timing_mask & (1 << kid)) { a = take_event(c); if (a) (void)hipEventRecord(a, c->st); }
     }
     ~Timer() {
-        if (c->timing && a) {
+        if (a) {
             hipEvent_t b = take_event(c);
-            if (b) { (void)hipEventRecord(b, c->st); c->pending.push_back({a, b, kid}); }
+            if (b) {
+                (void)hipEventRecord(b, c->st);
+                (c->sink ? c->sink : &c->pending)->push_back({a, b, kid});
+            }
         }
     }
 };
 
+static void add_elapsed(mfa_ctx *c, const TimedLaunch &t) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+        c->k_ms[t.kernel] += ms;
+        c->k_n[t.kernel] += 1;
+    }
+}
+
 static int collect_timing(mfa_ctx *c) {
-    if (c->pending.empty()) return 0;
+    if (c->pending.empty() && c->tring_unread == 0) return 0;
     HIPCHK(hipStreamSynchronize(c->st));
     for (auto &t : c->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
-            c->k_ms[t.kernel] += ms;
-            c->k_n[t.kernel] += 1;
-        }
+        add_elapsed(c, t);
         c->ev_pool.push_back(t.a);
         c->ev_pool.push_back(t.b);
     }
     c->pending.clear();
+    // the timing graphs replayed since the last read, oldest first
+    const int R = (int)c->tring.size();
+    for (int q = 0; q < c->tring_unread; ++q) {
+        const int g = ((c->tring_next - c->tring_unread + q) % R + R) % R;
+        for (auto &t : c->tring[g].ev) add_elapsed(c, t);
+    }
+    c->tring_unread = 0;
     return 0;
 }
 
@@ -197,6 +223,12 @@ static void free_all(mfa_ctx *c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->gexec_k) (void)hipGraphExecDestroy(c->gexec_k);
     if (c->graph_k) (void)hipGraphDestroy(c->graph_k);
+    for (auto &tg : c->tring) {
+        if (tg.ge) (void)hipGraphExecDestroy(tg.ge);
+        if (tg.g) (void)hipGraphDestroy(tg.g);
+        for (auto &t : tg.ev) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    }
+    c->tring.clear();
     if (c->st) (void)hipStreamDestroy(c->st);
 }
 
@@ -252,7 +284,12 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
     }
     d.r_cut_dpd_2 = p.r_cut_dpd_2;
     if (p.dpd_cut_off && !(p.r_cut_dpd_2 > 0.0)) return fail(MFA_E_ARG, "dpd_cut_off needs r_cut_dpd_2 > 0");
-    if (p.dpd_cut_off && sqrt(p.r_cut_dpd_2) > rc) return fail(MFA_E_ARG, "DPD cutoff exceeds the cell size");
+    // the cells and the Verlet list are complete between rebuilds for pairs
+    // within r_cut_max (every bead stays within skin/2 of its build position),
+    // so a DPD range beyond it would lose pairs: init_params.f90:801 folds
+    // the DPD cutoff into r_cut_max, and a caller must do the same
+    if (p.thermostat == 1 && p.dpd_cut_off && sqrt(p.r_cut_dpd_2) > p.r_cut_max)
+        return fail(MFA_E_ARG, "DPD cutoff exceeds r_cut_max (fold it into r_cut_max as init_params.f90:801 does)");
     d.sig = p.sig; d.friction = p.friction;
     d.r_2_min_init = p.r_2_min_init; d.r_2_min_time = p.r_2_min_time;
     for (int t = 0; t < 4; ++t) {
@@ -328,7 +365,9 @@ static int alloc_list(mfa_ctx *c) {
     return 0;
 }
 
-static void enqueue_rebuild(mfa_ctx *c) {
+// keep_frc: the permute carries the total force with the beads (every
+// rebuild except the fused step's, whose pair kernel overwrites the force)
+static void enqueue_rebuild(mfa_ctx *c, int keep_frc) {
     Timer t(c, K_REBUILD);
     const int g = std::min(GATED_GRID, std::max(1, c->n_blocks));
     const int gc = std::min(GATED_GRID, (c->C_tot + BLK - 1) / BLK);
@@ -345,10 +384,10 @@ static void enqueue_rebuild(mfa_ctx *c) {
            (const int *)c->d_bin_key, (const int *)c->d_key_tmp, c->d_perm, c->S_tot);
     if (c->prec == MFA_PREC_MIXED)
         LAUNCH((k_permute<true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot,
-               c->d_permute_done, c->d_rebuild_count, c->n_sys);
+               c->d_permute_done, c->d_rebuild_count, c->n_sys, keep_frc);
     else
         LAUNCH((k_permute<false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_perm, c->d_bin_cell, c->S_tot,
-               c->d_permute_done, c->d_rebuild_count, c->n_sys);
+               c->d_permute_done, c->d_rebuild_count, c->n_sys, keep_frc);
     if (c->B.nl_cap > 0) {
         if (c->prec == MFA_PREC_MIXED)
             LAUNCH((k_nlist<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange);
@@ -398,7 +437,7 @@ static void enqueue_bonded(mfa_ctx *c) {
 
 static void enqueue_step(mfa_ctx *c) {
     enqueue_vv1(c);
-    enqueue_rebuild(c);
+    enqueue_rebuild(c, 0);
     enqueue_pair(c);
     bool bend = false;
     for (const SysDev &d : c->hs) bend = bend || d.bending || d.bending_melt;
@@ -432,11 +471,20 @@ static int check_ctx(mfa_ctx *c, bool need_state = true) {
     return 0;
 }
 
+// (callers synchronise the stream first, so no replay is in flight; the
+// timing graphs' events return to the pool after their last read)
 static void invalidate_graph(mfa_ctx *c) {
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     if (c->gexec_k) { (void)hipGraphExecDestroy(c->gexec_k); c->gexec_k = nullptr; }
     if (c->graph_k) { (void)hipGraphDestroy(c->graph_k); c->graph_k = nullptr; }
+    for (auto &tg : c->tring) {
+        if (tg.ge) (void)hipGraphExecDestroy(tg.ge);
+        if (tg.g) (void)hipGraphDestroy(tg.g);
+        for (auto &t : tg.ev) { c->ev_pool.push_back(t.a); c->ev_pool.push_back(t.b); }
+    }
+    c->tring.clear();
+    c->tring_next = c->tring_unread = 0;
 }
 
 // Captures `steps` consecutive steps into one executable graph.  Every step
@@ -534,7 +582,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     if (precision == MFA_PREC_MIXED) {
         ALLOC(c->B.posf, 2 * (size_t)c->S_tot + NL_UNROLL); ALLOC(c->B.velf, 2 * (size_t)c->S_tot);
     }
-    ALLOC(c->B.frc, c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, n_sys); ALLOC(c->B.rebuild, 1);
+    ALLOC(c->B.frc, 2 * (size_t)c->S_tot); ALLOC(c->B.where, c->N_tot); ALLOC(c->B.parity, n_sys); ALLOC(c->B.rebuild, 1);
     ALLOC(c->B.chan_rb, n_sys);
     ALLOC(c->B.flags, 8); ALLOC(c->B.t_idx, 1); ALLOC(c->B.t_end, 1); ALLOC(c->B.store_frc, 1);
     ALLOC(c->d_sys, n_sys); ALLOC(c->d_blk_sys, c->n_blocks); ALLOC(c->d_cell_sys, c->C_tot);
@@ -615,7 +663,7 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
 
 static int ensure_cells(mfa_ctx *c) {
     // A rebuild is pending after set_state; run it (it is a no-op otherwise).
-    enqueue_rebuild(c);
+    enqueue_rebuild(c, 1);
     hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
     return 0;
 }
@@ -660,7 +708,7 @@ int mfa_hip_verlet_positions(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     ensure_cells(c);
     enqueue_vv1(c);
-    enqueue_rebuild(c);
+    enqueue_rebuild(c, 1);
     hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
     HIPCHK(hipGetLastError());
     return 0;
@@ -679,7 +727,7 @@ int mfa_hip_check_skin(mfa_ctx *c, int32_t *f_skin) {
 
 int mfa_hip_fluid_fluid(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
-    enqueue_rebuild(c);       // pending rebuild after set_state / verlet_positions (gated)
+    enqueue_rebuild(c, 1);    // pending rebuild after set_state / verlet_positions (gated)
     if (c->prec == MFA_PREC_MIXED) LAUNCH(k_velf_sync, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
     enqueue_pair(c);
     HIPCHK(hipGetLastError());
@@ -749,12 +797,46 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
     if (n_steps < 0) return fail(MFA_E_ARG, "md_steps: negative step count");
     ensure_cells(c);
     if (n_steps > 0) hipLaunchKernelGGL(k_set_t_end, dim3(1), dim3(1), 0, c->st, c->B, (int)n_steps);
-    if (!use_graph || c->timing) {
+    if (!use_graph || (c->timing_mask && !c->timing_graphs)) {
         for (int i = 0; i < n_steps; ++i) enqueue_step(c);
         HIPCHK(hipGetLastError());
         return 0;
     }
     int i = 0;
+    if (c->timing_graphs) {
+        // pair-only timing: replay the ring of timing graphs, read their
+        // events after every full turn; the remainder runs eagerly, timed
+        if (c->tring.empty()) {
+            c->tring.resize(TIMING_RING);
+            int rc = 0;
+            for (int g = 0; g < TIMING_RING && !rc; ++g) {
+                c->sink = &c->tring[g].ev;
+                rc = capture_steps(c, GRAPH_STEPS, &c->tring[g].g, &c->tring[g].ge);
+                c->sink = nullptr;
+            }
+            if (rc) {
+                // event records not capturable here: time the pair kernel
+                // with eager launches instead (capture_steps dropped the ring)
+                (void)hipGetLastError();
+                c->timing_graphs = false;
+            }
+        }
+        if (!c->timing_graphs) {
+            for (int k = 0; k < n_steps; ++k) enqueue_step(c);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+        for (; i + GRAPH_STEPS <= n_steps; i += GRAPH_STEPS) {
+            if (c->tring_unread == TIMING_RING)
+                if (int rc = collect_timing(c)) return rc;
+            HIPCHK(hipGraphLaunch(c->tring[c->tring_next].ge, c->st));
+            c->tring_next = (c->tring_next + 1) % TIMING_RING;
+            c->tring_unread += 1;
+        }
+        for (; i < n_steps; ++i) enqueue_step(c);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     if (n_steps >= GRAPH_STEPS) {
         if (!c->gexec_k)
             if (int rc = capture_steps(c, GRAPH_STEPS, &c->graph_k, &c->gexec_k)) return rc;
@@ -763,6 +845,30 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
     if (i < n_steps && !c->gexec)
         if (int rc = capture_steps(c, 1, &c->graph, &c->gexec)) return rc;
     for (; i < n_steps; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    return 0;
+}
+
+int mfa_hip_prepare_graphs(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->timing_graphs) {
+        if (c->tring.empty()) {
+            c->tring.resize(TIMING_RING);
+            int rc = 0;
+            for (int g = 0; g < TIMING_RING && !rc; ++g) {
+                c->sink = &c->tring[g].ev;
+                rc = capture_steps(c, GRAPH_STEPS, &c->tring[g].g, &c->tring[g].ge);
+                c->sink = nullptr;
+            }
+            if (rc) { (void)hipGetLastError(); c->timing_graphs = false; }
+        }
+        return 0;
+    }
+    if (c->timing_mask) return 0;       // eager launches while every group is timed
+    if (!c->gexec_k)
+        if (int rc = capture_steps(c, GRAPH_STEPS, &c->graph_k, &c->gexec_k)) return rc;
+    if (!c->gexec)
+        if (int rc = capture_steps(c, 1, &c->graph, &c->gexec)) return rc;
     return 0;
 }
 
@@ -977,8 +1083,12 @@ int mfa_hip_profiles_get(mfa_ctx *c, int32_t isys, int32_t *n_obs, double *dens_
 
 int mfa_hip_set_timing(mfa_ctx *c, int32_t on) {
     if (int rc = check_ctx(c, false)) return rc;
+    if (on < 0 || on > 2) return fail(MFA_E_ARG, "set_timing: mode must be 0, 1 or 2");
     if (int rc = collect_timing(c)) return rc;
-    c->timing = on != 0;
+    const bool graphs = on == 2;
+    if (graphs != c->timing_graphs || (on != 0) != (c->timing_mask != 0)) invalidate_graph(c);
+    c->timing_mask = on == 1 ? (1 << K_NK) - 1 : on == 2 ? (1 << K_PAIR) : 0;
+    c->timing_graphs = graphs;
     for (int k = 0; k < K_NK; ++k) { c->k_ms[k] = 0.0; c->k_n[k] = 0; }
     return 0;
 }
@@ -1024,6 +1134,7 @@ int mfa_hip_set_dpd_queue(mfa_ctx *c, int32_t depth) {
 
 int mfa_hip_set_virial(mfa_ctx *c, int32_t on) {
     if (int rc = check_ctx(c, false)) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));      // no replay of the old graphs is in flight
     c->virial = on != 0;
     invalidate_graph(c);
     return 0;

@@ -393,7 +393,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     const double4 *__restrict__ pos = CUR(B.pos);
     const double4 *__restrict__ vel = CUR(B.vel);
     const int *__restrict__ cell_of = CUR(B.cell_of);
-    double4 *__restrict__ frc = B.frc;
+    double4 *__restrict__ frc = CUR(B.frc);
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;
     __shared__ double tab[80];
@@ -514,11 +514,9 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     double e = block_sum(a.e, red);
     double np = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = e; out[C_NPAIR] = np; }
-    if (VIR) {
-        for (int q = 0; q < 6; ++q) {
-            double wq = block_sum(a.w[q], red);
-            if (threadIdx.x == 0) out[C_WXX + q] = wq;
-        }
+    for (int q = 0; q < 6; ++q) {       // zero when the virial is off
+        double wq = VIR ? block_sum(a.w[q], red) : 0.0;
+        if (threadIdx.x == 0) out[C_WXX + q] = wq;
     }
 }
 
@@ -642,7 +640,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const float4 *__restrict__ posf = CUR(B.posf);
     const float4 *__restrict__ velf = CUR(B.velf);
     const int *__restrict__ cell_of = CUR(B.cell_of);
-    double4 *__restrict__ frc = B.frc;
+    double4 *__restrict__ frc = CUR(B.frc);
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;
     __shared__ float tf[96];
@@ -805,11 +803,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     double es = block_sum(a.e, red);
     double nps = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }
-    if (VIR) {
-        for (int q = 0; q < 6; ++q) {
-            double wq = block_sum(a.w[q], red);
-            if (threadIdx.x == 0) out[C_WXX + q] = wq;
-        }
+    for (int q = 0; q < 6; ++q) {       // zero when the virial is off
+        double wq = VIR ? block_sum(a.w[q], red) : 0.0;
+        if (threadIdx.x == 0) out[C_WXX + q] = wq;
     }
 }
 

@@ -85,6 +85,7 @@ struct BondOut { double fx, fy, fz, pot; };
 // FENE bond (a, b=a+1), intra_molec.f90:21-58: returns force_loc (the force
 // the reference adds to b and subtracts from a) and -0.5 ln(1-q).
 __device__ __forceinline__ BondOut fene(const SysDev &P, const double4 &ra, const double4 &rb, int *stretched) {
+#pragma clang fp contract(off)
     BondOut o;
     double dx = mic(ra.x - rb.x, P.L[0], P.invL[0]);
     double dy = mic(ra.y - rb.y, P.L[1], P.invL[1]);
@@ -104,6 +105,7 @@ struct BendOut { double fp[3], fn[3], v; };
 // Harmonic-angle bending centred on c, bending.f90:30-109 (including the
 // small-angle fallback :59-92 and the cos >= 1 guard :93-98).
 __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm, const double4 &rc, const double4 &rpn) {
+#pragma clang fp contract(off)
     BendOut o;
     double rp[3], rn[3];
     rp[0] = mic(rc.x - rm.x, P.L[0], P.invL[0]);
@@ -161,6 +163,7 @@ __device__ __forceinline__ double4 bead_at(const Bufs &B, size_t cur_off, int go
 // (centres ascending): +fn of centre m-1, -fp-fn of centre m, +fp of m+1.
 __device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, size_t cur_off, int g0, int nm, int m,
                                           const double4 ri, double3 &F, double &ev) {
+#pragma clang fp contract(off)
     // positions of the beads m-2 .. m+2 of the chain (absent ones are never used)
     double4 rm2 = ri, rm1 = ri, rp1 = ri, rp2 = ri;
     if (m >= 2) rm2 = bead_at(B, cur_off, g0 + m - 2);
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         const int o = g - P.part_base;
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
         guard(B, in_range(o, P.n_part), G_ORIG, k);
-        double4 f4 = B.frc[k];
+        double4 f4 = CUR(B.frc)[k];
         double3 F = make_double3(f4.x, f4.y, f4.z);
         const bool brush = o < P.pd;
         const int nm = brush ? P.n_mon : P.n_mon_d;
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
             if (MIXED && !FUSED) CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
         }
         f4.x = F.x; f4.y = F.y; f4.z = F.z;
-        if (store_f) B.frc[k] = f4;
+        if (store_f) CUR(B.frc)[k] = f4;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
     if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
@@ -486,11 +489,15 @@ __global__ __launch_bounds__(BLK) void k_rank(const SysDev *__restrict__ sys, co
 // Permutes the rebuilding channels into the other half.  The last block to
 // finish (every block has read the parities by then) flips those channels'
 // parities and counts the rebuild; `done` is its ticket counter, left at 0.
+// keep_frc: the total force moves with its bead (the state a caller reads
+// after verlet_positions holds the last step's force, as the reference's
+// particle-indexed force array does); the fused step passes 0, since its pair
+// kernel overwrites the force right after the rebuild.
 template <bool MIXED>
 __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
                                                 const int *__restrict__ perm, const int *__restrict__ bin_cell,
                                                 int n_slots, int *__restrict__ done, int *__restrict__ rebuild_count,
-                                                int n_sys) {
+                                                int n_sys, int keep_frc) {
     if (!*B.rebuild) return;
     for (int m = blockIdx.x * BLK + threadIdx.x; m < n_slots; m += gridDim.x * BLK) {
         const int si = blk_sys[m / BLK];
@@ -505,6 +512,7 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         NXT(B.pos)[m] = r;
         NXT(B.vel)[m] = v;
         NXT(B.acc)[m] = CUR(B.acc)[old];
+        if (keep_frc) NXT(B.frc)[m] = CUR(B.frc)[old];
         NXT(B.posb)[m] = r;                            // skin reference of this build
         const int cg = bin_cell[old];
         NXT(B.cell_of)[m] = cg;
@@ -550,7 +558,7 @@ __global__ __launch_bounds__(BLK) void k_upload(const SysDev *__restrict__ sys, 
     CUR(B.pos)[k] = make_double4(r0[3 * i], r0[3 * i + 1], r0[3 * i + 2], mkw(g, type[i]));
     CUR(B.vel)[k] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
     CUR(B.acc)[k] = a ? make_double4(a[3 * i], a[3 * i + 1], a[3 * i + 2], 0.0) : make_double4(0.0, 0.0, 0.0, 0.0);
-    B.frc[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+    CUR(B.frc)[k] = make_double4(0.0, 0.0, 0.0, 0.0);
     CUR(B.cell_of)[k] = -1;
     B.where[g] = k;
 }
@@ -563,7 +571,7 @@ __global__ __launch_bounds__(BLK) void k_download(const SysDev *__restrict__ sys
     if (i >= P.n_part) return;
     const int k = B.where[P.part_base + i];
     if (!guard(B, in_range(k, B.n_slots), G_WHERE, i)) return;
-    const double4 *src = WHICH == 0 ? CUR(B.pos) : WHICH == 1 ? CUR(B.vel) : WHICH == 2 ? CUR(B.acc) : B.frc;
+    const double4 *src = WHICH == 0 ? CUR(B.pos) : WHICH == 1 ? CUR(B.vel) : WHICH == 2 ? CUR(B.acc) : CUR(B.frc);
     const double4 q = src[k];
     out[3 * i] = q.x; out[3 * i + 1] = q.y; out[3 * i + 2] = q.z;
 }

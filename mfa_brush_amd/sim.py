@@ -163,6 +163,11 @@ class Simulation:
     def md_steps(self, n: int, graph: bool = False):
         self._c(self.lib.mfa_hip_md_steps(self.h, n, int(graph)), "md_steps")
 
+    def prepare_graphs(self):
+        """Capture the graphs md_steps(graph=True) replays in the current
+        timing mode, without running a step."""
+        self._c(self.lib.mfa_hip_prepare_graphs(self.h), "prepare_graphs")
+
     def synchronize(self):
         self._c(self.lib.mfa_hip_synchronize(self.h), "synchronize")
 
@@ -254,7 +259,9 @@ class Simulation:
         self._c(self.lib.mfa_hip_get_neighbor_list(self.h, C.byref(n)), "get_neighbor_list")
         return n.value
 
-    def set_timing(self, on: bool = True):
+    def set_timing(self, on=True):
+        """0/False off; 1/True every kernel group, eager launches; 2 the pair
+        kernel only, with graph replay (event records captured as nodes)."""
         self._c(self.lib.mfa_hip_set_timing(self.h, int(on)), "set_timing")
 
     def kernel_time(self, name: str):

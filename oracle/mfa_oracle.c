@@ -555,6 +555,41 @@ void orc_fluid_fluid_brute(const orc_params *p, const double *r0, const double *
         for (int i = 0; i < p->n_part; ++i) lgv_bead(p, v, a_type, i, step, noise, force);
 }
 
+/* Pair virial W = sum over interacting pairs i<j of r_ij (x) F_ij, with r_ij
+ * the minimum-image separation r_i - r_j and F_ij the total pair force on i
+ * (LJ, dissipative and random parts), components xx yy zz xy xz yz.  The
+ * reference accumulates a pressure tensor only in its bulk build
+ * (fluid_fluid.f90:156-166, dpd_forces.f90:118-127) and none in channel mode;
+ * this is the channel-mode virial of the device path.  Each pair goes through
+ * pair_body, so the pair force is exactly the one orc_fluid_fluid_brute adds:
+ * the LJ part lands in f_i, the DPD part in the scratch rows i and j, which
+ * are read back and cleared after every pair. */
+void orc_pair_virial(const orc_params *p, const double *r0, const double *v, const int *a_type,
+                     double r_2_min, uint64_t step, int noise, double W[6]) {
+    double *tmp = (double *)calloc(3 * (size_t)p->n_part, sizeof(double));
+    double e = 0.0;
+    for (int q = 0; q < 6; ++q) W[q] = 0.0;
+    for (int i = 0; i < p->n_part; ++i) {
+        for (int j = i + 1; j < p->n_part; ++j) {
+            double fi[3] = {0.0, 0.0, 0.0};
+            pair_body(p, r0, v, a_type, i, j, r_2_min, step, noise, fi, tmp, &e);
+            double f[3], dr[3];
+            for (int d = 0; d < 3; ++d) {
+                f[d] = fi[d] + tmp[3 * i + d];
+                tmp[3 * i + d] = 0.0;
+                tmp[3 * j + d] = 0.0;
+                dr[d] = r0[3 * i + d] - r0[3 * j + d];
+            }
+            if (f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0) continue;
+            dr[0] = mic(dr[0], p->boundary[0], p->inv_boundary[0]);
+            dr[1] = mic(dr[1], p->boundary[1], p->inv_boundary[1]);
+            W[0] += dr[0] * f[0]; W[1] += dr[1] * f[1]; W[2] += dr[2] * f[2];
+            W[3] += dr[0] * f[1]; W[4] += dr[0] * f[2]; W[5] += dr[1] * f[2];
+        }
+    }
+    free(tmp);
+}
+
 /* ------------------------------------------------------------------------
  * Walls, bonds, bending, body force
  * ---------------------------------------------------------------------- */

@@ -91,6 +91,9 @@ int  orc_pair_stats_cells(const orc_params *p, const double *r0, const int *a_ty
                           int32_t *count, uint64_t *hash);
 void orc_fluid_fluid_brute(const orc_params *p, const double *r0, const double *v, const int *a_type,
                            double r_2_min, uint64_t step, int noise, double *force, double *v_ff);
+/* pair virial sum_{i<j} r_ij (x) F_ij (xx yy zz xy xz yz), all pair-force parts */
+void orc_pair_virial(const orc_params *p, const double *r0, const double *v, const int *a_type,
+                     double r_2_min, uint64_t step, int noise, double W[6]);
 double orc_fluid_wall(const orc_params *p, const double *r0, const int *a_type, double *force);
 double orc_wall_wall(const orc_params *p, double sigma_w);
 double orc_intra_molec(const orc_params *p, const double *r0, double *force, int *n_stretched);

@@ -1,0 +1,134 @@
+"""GPU parity of the virial, the DPD range beyond the LJ range, the force
+carried through a rebuild, and the mixed path's force error at full size.
+
+Bars: FP64 within 1e-10 of the oracle (relative to the largest diagonal
+virial component, or to the largest force); mixed within 1e-5 (max) and
+2e-6 (RMS) of the largest force, per workload."""
+import numpy as np
+import pytest
+
+from mfa_brush_amd import configs
+from mfa_brush_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _perturbed(builder, sigma=0.05, seed=5, **kw):
+    p, t, r0, v = builder(**kw)
+    rng = np.random.default_rng(seed)
+    r0 = r0 + rng.normal(0, sigma, r0.shape)
+    r0[:, 0] %= p.boundary[0]
+    r0[:, 1] %= p.boundary[1]
+    return p, t, r0, v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("thermostat", [False, True])
+def test_pair_virial_vs_oracle(oracle_mod, precision, thermostat):
+    """sum over pairs of r_ij (x) F_ij (LJ, then LJ + dissipative + random
+    DPD at the same Philox noise) from the pair kernel against the oracle."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=thermostat, rc_dpd=1.0)
+    with Simulation(p, precision=precision, t_idx0=17) as s:
+        s.set_virial(True)
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        w = s.energies()["virial"]
+        s.set_virial(False)
+        s.fluid_fluid()
+        assert not s.energies()["virial"].any()
+    wo = oracle_mod.Oracle(p).pair_virial(r0, v, t, step=17, noise=1)
+    scale = np.abs(wo[:3]).max()
+    err = np.abs(w - wo).max() / scale
+    print("virial %s thermostat=%s: %s vs %s, err %.2e" % (precision, thermostat, w, wo, err))
+    assert err < (1e-10 if precision == "fp64" else 1e-5), err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dpd_beyond_lj_range(oracle_mod, precision):
+    """Solvent-solvent LJ cut at 0.9 under a DPD cutoff of 1: the pairs
+    between 0.9 and 1 take only the DPD term, with their own distance."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
+    r2 = np.asarray(p.range_2).reshape(4, 4).T.copy()
+    r2[2, 2] = 0.81
+    col = lambda m: np.ascontiguousarray(m.T.reshape(-1))
+    p = p.replace(range_2=col(r2), inv_range_2=col(np.where(r2 > 0, 1 / np.where(r2 > 0, r2, 1), 0)))
+    with Simulation(p, precision=precision, t_idx0=3) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+    fo, _ = oracle_mod.Oracle(p).fluid_fluid(r0, v, t, step=3, noise=1)
+    err = np.linalg.norm(f - fo, axis=1).max() / np.linalg.norm(fo, axis=1).max()
+    assert err < (1e-11 if precision == "fp64" else 1e-5), err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_force_follows_beads_through_rebuild(precision):
+    """verlet_positions re-sorts the beads by cell (every step without a
+    skin); the force read afterwards is still the last evaluation's force of
+    each bead, as the reference's particle-indexed force array is."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.0)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(3)
+        f0 = s.get_state()["force"]
+        s.verlet_positions()
+        f1 = s.get_state()["force"]
+        assert s.flags()["rebuilds"] >= 4
+    assert np.abs(f0).max() > 1.0
+    assert np.array_equal(f0, f1)
+
+
+@pytest.mark.parametrize("workload", ["c2", "c3"])
+def test_full_size_force_error(oracle_mod, workload):
+    """Thermostat off, full size (998,784-bead brush channel; 1.2M-bead melt
+    with bending): both paths against the oracle's linked-cell forces.  The
+    mixed path's error is stated per workload: max and RMS of
+    |F - F_oracle| over beads, relative to the largest (RMS) oracle force."""
+    if workload == "c2":
+        p, t, r0, v = _perturbed(configs.channel_c2, sigma=0.03, thermostat=False, rc_dpd=1.0)
+    else:
+        p, t, r0, v = _perturbed(configs.channel_melt, sigma=0.03, thermostat=False, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    fo, eo = oracle_mod.Oracle(p, omp=True).fluid_fluid(r0, v, t, noise=0)
+    no = np.linalg.norm(fo, axis=1)
+    for precision in ("fp64", "mixed"):
+        with Simulation(p, precision=precision) as s:
+            s.set_state(r0, v, t)
+            s.fluid_fluid()
+            f = s.get_state()["force"]
+            e = s.energies()["v_fluid_fluid"]
+        d = np.linalg.norm(f - fo, axis=1)
+        err_max = d.max() / no.max()
+        err_rms = np.sqrt((d ** 2).mean() / (no ** 2).mean())
+        err_e = abs(e - eo) / abs(eo)
+        print("%s %s: force err max %.2e rms %.2e, energy rel err %.2e (max|F| %.1f)"
+              % (workload, precision, err_max, err_rms, err_e, no.max()))
+        if precision == "fp64":
+            assert err_max < 1e-11 and err_e < 1e-11
+        else:
+            assert err_max < 1e-5 and err_rms < 2e-6 and err_e < 1e-5
+
+
+def test_pair_timing_graphs_same_bits():
+    """Pair-kernel timing with the ring of timing graphs (event records as
+    graph nodes) times every pair launch and leaves the trajectory's bits
+    unchanged."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.4)
+    outs = []
+    for mode in (0, 2):
+        with Simulation(p, precision="mixed") as s:
+            s.set_state(r0, v, t)
+            s.set_timing(mode)
+            s.prepare_graphs()
+            s.md_steps(45, graph=True)          # 5 graph replays (one more than the ring) + 5 eager steps
+            s.synchronize()
+            ms, n = s.kernel_time("pair")
+            outs.append(s.get_state())
+        if mode == 2:
+            assert n == 45 and ms > 0.0, (n, ms)
+        else:
+            assert n == 0
+    for k in ("r0", "v", "a", "force"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k

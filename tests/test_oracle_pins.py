@@ -1,0 +1,299 @@
+"""Pins of the oracle branches that the reference build in oracle/_ref cannot
+reach (its default control_simulation.h undefines DPD_CUT_OFF and has no
+channel-mode virial) and of the melt-chain bending range.  Each is checked
+against what the mathematics fixes: closed forms on two beads, the
+virial's scaling and shear derivatives of the LJ energy, and the identity
+sum_pairs r_ij (x) F_ij = sum_i r_i (x) F_i for a cluster that no pair
+crosses the periodic boundary of.  A plausible slip (a dropped w factor, a
+DPD term on a head pair, a stale 1/r, the melt range shifted by one chain,
+a transposed virial component) fails one of them."""
+import math
+
+import numpy as np
+import pytest
+
+from mfa_brush_amd import configs
+from mfa_brush_amd.params import Flags
+
+
+def _two(p, dr, ti, tj, v=None):
+    r0 = np.array([[6.0, 6.0, 8.0], [6.0, 6.0, 8.0]])
+    r0[1] += np.asarray(dr, float)
+    if v is None:
+        v = np.zeros((2, 3))
+    return r0, np.array([ti, tj], np.int32), np.asarray(v, float)
+
+
+def _pair_params(rc_dpd=1.0, **kw):
+    p = configs.channel_params((12.0, 12.0, 16.0), 1, 0, 2, rc_dpd=rc_dpd, **kw)
+    return p
+
+
+def _dpd_part(o, r0, t, v, step, noise):
+    f_lj, _ = o.fluid_fluid_brute(r0, np.zeros_like(v), t, noise=0)
+    f, _ = o.fluid_fluid_brute(r0, v, t, step=step, noise=noise)
+    return f - f_lj, f_lj
+
+
+# ------------------------------------------- DPD_CUT_OFF (rc_dpd = 1) ----
+@pytest.mark.parametrize("ti,tj", [(3, 3), (2, 3), (2, 2)])
+def test_dpd_cutoff_dissipative_closed_form(oracle_mod, ti, tj):
+    """r = 0.8 < rc_dpd = 1 (fluid_fluid.f90:217-262, dpd_forces.f90:47-52):
+    F_D on bead 0 = -gamma w^2 (dv . rhat) rhat with w = 1 - r/rc_dpd, and
+    -F_D on bead 1; the LJ part is unchanged by the velocities."""
+    p = _pair_params().replace(sig=0.0)                # dissipative part only
+    o = oracle_mod.Oracle(p)
+    d = np.array([0.48, -0.64, 0.0])                  # |d| = 0.8, off-axis
+    r0, t, v = _two(p, d, ti, tj, v=[[0.3, -0.2, 0.1], [-0.5, 0.4, 0.25]])
+    fd, _ = _dpd_part(o, r0, t, v, 5, 1)
+    r = 0.8
+    rhat = (r0[0] - r0[1]) / r
+    w = 1.0 - r / 1.0
+    want = -p.friction * w * w * (v[0] - v[1]).dot(rhat) * rhat
+    assert np.allclose(fd[0], want, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(fd[1], -fd[0])
+
+
+def test_dpd_cutoff_random_closed_form(oracle_mod):
+    """Random part at r = 0.8: sig (1 - r/rc_dpd) g rhat, g the Philox normal
+    of the unordered pair at this step; exactly antisymmetric."""
+    lib = oracle_mod.load()
+    p = _pair_params(gamma=0.0).replace(sig=2.5)
+    o = oracle_mod.Oracle(p)
+    r0, t, v = _two(p, [0.0, 0.0, -0.8], 3, 3)
+    fr, _ = _dpd_part(o, r0, t, v, 23, 1)
+    g = lib.orc_pair_gauss(p.seed, 23, 0, 1)
+    want = 2.5 * (1.0 - 0.8) * g * np.array([0.0, 0.0, 1.0])
+    assert np.allclose(fr[0], want, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(fr[1], -fr[0])
+    assert abs(g) > 1e-3                               # the draw is not degenerate
+
+
+@pytest.mark.parametrize("ti,tj", [(1, 3), (1, 2), (3, 1)])
+def test_dpd_cutoff_excludes_heads(oracle_mod, ti, tj):
+    """A pair with a grafted head (type 1) takes no DPD term under
+    DPD_CUT_OFF (fluid_fluid.f90:223-236), only its LJ force."""
+    p = _pair_params()
+    o = oracle_mod.Oracle(p)
+    r0, t, v = _two(p, [0.8, 0.0, 0.0], ti, tj, v=[[1.0, 0.5, 0.0], [-1.0, 0.0, 0.3]])
+    fd, f_lj = _dpd_part(o, r0, t, v, 9, 1)
+    assert not fd.any()
+    assert np.abs(f_lj).max() > 1.0                    # the LJ part is there
+
+
+@pytest.mark.parametrize("r", [1.05, 1.6, 2.4])
+def test_dpd_cutoff_lj_only_beyond_rc_dpd(oracle_mod, r):
+    """1 < r < 2.5: LJ only (the DPD range is the DPD cutoff, not the LJ one)."""
+    p = _pair_params()
+    o = oracle_mod.Oracle(p)
+    r0, t, v = _two(p, [r, 0.0, 0.0], 3, 3, v=[[1.0, 0.5, 0.0], [-1.0, 0.0, 0.3]])
+    fd, f_lj = _dpd_part(o, r0, t, v, 9, 1)
+    assert not fd.any()
+    s6 = r ** -6
+    assert f_lj[0, 0] == pytest.approx(-24.0 * (2 * s6 * s6 - s6) / r, rel=1e-12)
+
+
+def test_dpd_beyond_lj_range_uses_own_distance(oracle_mod):
+    """A pair inside rc_dpd but outside its LJ range (solvent-solvent LJ cut
+    at 0.9 here) takes the DPD term with its own 1/r: the force is the closed
+    form along its own rhat, and no LJ force.  (The reference would reuse
+    the previous pair's 1/r, fluid_fluid.f90:253-256; the restatement
+    deliberately does not.)"""
+    p = _pair_params()
+    r2 = np.asarray(p.range_2).reshape(4, 4).T.copy()
+    r2[2, 2] = 0.81
+    col = lambda m: np.ascontiguousarray(m.T.reshape(-1))
+    p = p.replace(range_2=col(r2), inv_range_2=col(np.where(r2 > 0, 1 / np.where(r2 > 0, r2, 1), 0)))
+    o = oracle_mod.Oracle(p)
+    lib = oracle_mod.load()
+    d = np.array([0.0, 0.57, 0.76])                   # r = 0.95
+    r0, t, v = _two(p, d, 3, 3, v=[[0.2, 0.7, -0.4], [0.0, -0.3, 0.6]])
+    f, e = o.fluid_fluid_brute(r0, v, t, step=4, noise=1)
+    r = 0.95
+    rhat = (r0[0] - r0[1]) / r
+    w = 1.0 - r
+    g = lib.orc_pair_gauss(p.seed, 4, 0, 1)
+    want = (p.sig * w * g - p.friction * w * w * (v[0] - v[1]).dot(rhat)) * rhat
+    assert e == 0.0
+    assert np.allclose(f[0], want, rtol=1e-12, atol=1e-15)
+    assert np.allclose(f[1], -want, rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------- bending_melt ----
+def _melt_system(oracle_mod, k_bend=5.0):
+    """4 brush chains of 5 and 6 melt chains of 7 beads, crumpled."""
+    p = configs.channel_params((12.0, 12.0, 14.0), 5, 4, 6, thermostat=False, bending=True, k_bend=k_bend)
+    p = p.replace(n_mon_d=7, flags=Flags(**{**p.flags.__dict__, "bending_melt": True}))
+    rng = np.random.default_rng(11)
+    r0, t = [], []
+    for c in range(4):
+        base = np.array([1.5 + 3.0 * c, 2.0, 1.2])
+        for m in range(5):
+            r0.append(base + [0.2 * rng.standard_normal(), 0.2 * rng.standard_normal(), 0.95 * m])
+            t.append(1 if m == 0 else 2)
+    for c in range(6):
+        x = np.array([2.0 + 1.7 * c, 7.0, 7.0])
+        for m in range(7):
+            r0.append(x.copy())
+            t.append(3)
+            step = rng.normal(0, 1, 3)
+            x = x + 0.95 * step / np.linalg.norm(step)
+    r0 = np.array(r0)
+    r0[:, :2] %= 12.0
+    return p, oracle_mod.Oracle(p), r0, np.array(t, np.int32)
+
+
+def test_bending_melt_force_is_minus_gradient(oracle_mod):
+    p, o, r0, t = _melt_system(oracle_mod)
+    f, e = o.bending(r0, 1)
+    assert np.abs(f[:20]).max() == 0.0                 # brush beads untouched by the melt term
+    assert np.abs(f[20:]).max() > 0.1
+    h = 1e-6
+    for i in range(20, len(r0)):
+        for d in range(3):
+            rp = r0.copy(); rp[i, d] += h
+            rm = r0.copy(); rm[i, d] -= h
+            g = (o.bending(rp, 1)[1] - o.bending(rm, 1)[1]) / (2 * h)
+            assert -g == pytest.approx(f[i, d], rel=2e-5, abs=2e-6)
+
+
+def test_bending_melt_range_and_right_angle(oracle_mod):
+    """Only the melt chains bend under bending_melt, each over its own beads:
+    a melt chain of three beads with a right angle carries energy
+    k (pi/2)^2 / 2, the end beads get force k (pi/2) / |b| perpendicular to
+    their bond, the chain's forces sum to zero, straight chains get none.
+    (The reference indexes the melt chain without its offset,
+    bending_melt.f90:43; the restatement uses the chain's own beads.)"""
+    p = configs.channel_params((12.0, 12.0, 14.0), 3, 2, 3, thermostat=False, bending=True, k_bend=5.0)
+    p = p.replace(n_mon_d=3, flags=Flags(**{**p.flags.__dict__, "bending_melt": True}))
+    o = oracle_mod.Oracle(p)
+    brush = [[1, 1, 2.0], [1, 1, 3.0], [2, 1, 3.0], [5, 5, 10.0], [5, 5, 9.0], [5, 6, 9.0]]
+    melt = [[3, 3, 6.0], [3, 3, 7.0], [3, 3, 8.0],          # straight
+            [8, 8, 6.0], [8, 8, 7.0], [9, 8, 7.0],          # right angle
+            [6, 9, 4.0], [6, 9, 5.0], [6, 9, 6.0]]          # straight
+    r0 = np.array(brush + melt, float)
+    f, e = o.bending(r0, 1)
+    assert e == pytest.approx(0.5 * 5.0 * (math.pi / 2) ** 2, rel=1e-12)
+    assert not f[:6].any() and np.abs(f[6:9]).max() < 1e-12 and np.abs(f[12:15]).max() < 1e-12
+    fc = f[9:12]
+    assert np.abs(fc.sum(0)).max() < 1e-12
+    for end, nb in ((0, 1), (2, 1)):
+        b = r0[9 + end] - r0[9 + nb]
+        assert abs(fc[end].dot(b)) < 1e-12
+        assert np.linalg.norm(fc[end]) == pytest.approx(5.0 * math.pi / 2 / np.linalg.norm(b), rel=1e-12)
+    # the brush chains with their own right angles stay with bending(…, 0)
+    fb, eb = o.bending(r0, 0)
+    assert eb == pytest.approx(2 * 0.5 * 5.0 * (math.pi / 2) ** 2, rel=1e-12) and not fb[6:].any()
+
+
+# ---------------------------------------------------------- pair virial ----
+def _lj_fluid(n, box, y_slab=None, seed=3, dmin=0.88):
+    rng = np.random.default_rng(seed)
+    pts = []
+    lo = np.array([0.0, 0.0, 1.0])
+    hi = np.array([box[0], box[1], box[2] - 1.0])
+    if y_slab is not None:
+        lo[1], hi[1] = y_slab
+    while len(pts) < n:
+        q = rng.uniform(lo, hi)
+        ok = True
+        for r in pts:
+            d = q - r
+            d[:2] -= np.round(d[:2] / np.asarray(box[:2])) * np.asarray(box[:2])
+            if d.dot(d) < dmin * dmin:
+                ok = False
+                break
+        if ok:
+            pts.append(q)
+    t = np.where(np.arange(n) % 3 == 0, 2, 3).astype(np.int32)
+    return np.array(pts), t
+
+
+def _vparams(box, n, thermostat=False, rc_dpd=1.0):
+    return configs.channel_params(box, 1, 0, n, thermostat=thermostat, rc_dpd=rc_dpd)
+
+
+def test_virial_two_beads_closed_form(oracle_mod):
+    p = _vparams((12.0, 12.0, 16.0), 2)
+    o = oracle_mod.Oracle(p)
+    d = np.array([-0.7, 0.5, 0.6])
+    r0, t, v = _two(p, d, 3, 3)
+    w = o.pair_virial(r0, v, t, noise=0)
+    r01 = -d
+    r2 = r01.dot(r01)
+    s6 = r2 ** -3
+    fmag = 24.0 * (2 * s6 * s6 - s6) / r2              # F_0 = fmag * r01
+    want = [r01[a] * r01[b] * fmag for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    assert np.allclose(w, want, rtol=1e-12)
+
+
+def test_virial_two_beads_across_periodic_boundary(oracle_mod):
+    """The separation is the minimum image: a pair split by the x boundary
+    gives the virial of the same pair in the box interior."""
+    p = _vparams((12.0, 12.0, 16.0), 2)
+    o = oracle_mod.Oracle(p)
+    r0 = np.array([[0.3, 5.0, 8.0], [11.5, 5.4, 8.3]])
+    r1 = np.array([[6.3, 5.0, 8.0], [5.5, 5.4, 8.3]])
+    t = np.array([3, 3], np.int32)
+    w0 = o.pair_virial(r0, np.zeros((2, 3)), t, noise=0)
+    w1 = o.pair_virial(r1, np.zeros((2, 3)), t, noise=0)
+    assert np.abs(w0).max() > 0.1 and np.allclose(w0, w1, rtol=1e-12, atol=1e-14)
+
+
+def test_virial_is_energy_derivative_under_strain(oracle_mod):
+    """W_aa = -dV/d(lambda_a) under r_a -> lambda_a r_a (x and y with the box,
+    z inside the walls), W_xz = -dV/dg under x -> x + g z, W_yz likewise,
+    W_xy = -dV/dg under x -> x + g y for a fluid no pair of which wraps in y."""
+    box = (11.0, 11.0, 11.0)
+    r0, t = _lj_fluid(260, box, y_slab=(2.6, 8.4))
+    p = _vparams(box, len(r0))
+    o = oracle_mod.Oracle(p)
+    v = np.zeros_like(r0)
+    w = o.pair_virial(r0, v, t, noise=0)
+    h = 1e-6
+
+    def energy(r, bx):
+        return oracle_mod.Oracle(p.replace(boundary=tuple(bx))).check_fluid_fluid(r, t)
+
+    def d_scale(a):
+        out = []
+        for s in (1 + h, 1 - h):
+            r = r0.copy(); r[:, a] *= s
+            bx = list(box)
+            if a < 2:
+                bx[a] *= s
+            out.append(energy(r, bx))
+        return (out[0] - out[1]) / (2 * h)
+
+    def d_shear(a, b):
+        out = []
+        for g in (h, -h):
+            r = r0.copy(); r[:, a] += g * r0[:, b]
+            out.append(energy(r, box))
+        return (out[0] - out[1]) / (2 * h)
+
+    want = [-d_scale(0), -d_scale(1), -d_scale(2), -d_shear(0, 1), -d_shear(0, 2), -d_shear(1, 2)]
+    scale = np.abs(w[:3]).max()
+    assert scale > 10.0
+    assert np.allclose(w, want, rtol=0, atol=1e-6 * scale), (w, want)
+    assert np.abs(w[3:]).max() > 1e-3 * scale          # off-diagonals are not trivially zero here
+
+
+@pytest.mark.parametrize("noise", [0, 1])
+def test_virial_cluster_identity_with_dpd(oracle_mod, noise):
+    """sum_pairs r_ij (x) F_ij = sum_i r_i (x) F_i for a cluster that no pair
+    crosses a periodic boundary of, with the DPD dissipative (and random)
+    forces included: the pair force of the virial is the one the force
+    routine adds."""
+    box = (14.0, 14.0, 12.0)
+    r0, t = _lj_fluid(300, (8.0, 8.0, 12.0), seed=8)
+    r0[:, :2] += 3.0
+    p = _vparams(box, len(r0), thermostat=True, rc_dpd=1.0)
+    o = oracle_mod.Oracle(p)
+    v = np.random.default_rng(2).normal(0, 1, r0.shape)
+    f, _ = o.fluid_fluid_brute(r0, v, t, step=7, noise=noise)
+    want = [(r0[:, a] * f[:, b]).sum() for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    w = o.pair_virial(r0, v, t, step=7, noise=noise)
+    f_lj, _ = o.fluid_fluid_brute(r0, np.zeros_like(v), t, noise=0)
+    assert np.abs(f - f_lj).max() > 0.1                # DPD forces present
+    assert np.allclose(w, want, rtol=0, atol=1e-10 * np.abs(want).max())
