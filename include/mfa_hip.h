@@ -135,8 +135,8 @@ int  mfa_hip_verlet_velocities(mfa_ctx *ctx);      /* verlet_velocities.f90:18-1
  * that mfa_hip_get_state returns afterwards is the total force of the last
  * step (the steps before it do not write it back). */
 int  mfa_hip_md_steps(mfa_ctx *ctx, int32_t n_steps, int32_t use_graph);
-/* Captures the graphs md_steps(use_graph=1) replays in the current timing
- * mode, without running a step (a timed region then pays no capture). */
+/* Captures the graphs md_steps(use_graph=1) replays, without running a step
+ * (a timed region then pays no capture); a no-op while timing. */
 int  mfa_hip_prepare_graphs(mfa_ctx *ctx);
 int  mfa_hip_synchronize(mfa_ctx *ctx);
 
@@ -166,11 +166,9 @@ int  mfa_hip_get_cells(mfa_ctx *ctx, int32_t isys, int32_t *n_cells, int32_t *ce
 
 /* ---- timing ------------------------------------------------------------
  * Mode 1: each fused-step kernel group (vv1, rebuild, pair, bonded) is
- * bracketed by hipEvents on the context's stream, and md_steps launches
- * eagerly.  Mode 2: only the pair kernel is bracketed, and md_steps keeps
- * replaying graphs: a ring of captured graphs carries the event records as
- * graph nodes, read after every turn of the ring.  Mode 0: off.  Totals
- * are in milliseconds. */
+ * bracketed by hipEvents on the context's stream.  Mode 2: only the pair
+ * kernel is (two event records per step instead of eight).  Mode 0: off.
+ * While timing, md_steps launches eagerly.  Totals are in milliseconds. */
 int  mfa_hip_set_timing(mfa_ctx *ctx, int32_t on);
 int  mfa_hip_get_kernel_time(mfa_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
 int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off); waits

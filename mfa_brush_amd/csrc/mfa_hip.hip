@@ -43,7 +43,6 @@ namespace {
 // step on 256 replicas, 3.5M beads) and cost nothing measurable when idle
 constexpr int GATED_GRID = 2048;
 constexpr int GRAPH_STEPS = 8;       // steps per replayed graph (plus a one-step graph for the rest)
-constexpr int TIMING_RING = 4;       // timing graphs replayed in turn between two reads of their events
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -82,14 +81,10 @@ struct mfa_ctx {
     std::vector<char> uploaded;
     bool virial = false;
     int nl_request = -1;          // Verlet-list capacity: -1 automatic, 0 off, > 0 entries per bead
-    // timing: timing_mask has bit k set when kernel group k is bracketed by
-    // events; sink (non-null while a timing graph is captured) receives the
-    // captured event pairs instead of `pending`
+    // timing: bit k of timing_mask set when kernel group k is bracketed by events
     int timing_mask = 0;
-    bool timing_graphs = false;          // pair-only mode: graph replay with event nodes
     std::vector<hipEvent_t> ev_pool;
     std::vector<TimedLaunch> pending;
-    std::vector<TimedLaunch> *sink = nullptr;
     double k_ms[K_NK] = {0, 0, 0, 0};
     long long k_n[K_NK] = {0, 0, 0, 0};
     // graph
@@ -97,13 +92,6 @@ struct mfa_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec_k = nullptr;    // GRAPH_STEPS steps
     hipGraph_t graph_k = nullptr;
-    // timing graphs: TIMING_RING captured graphs of GRAPH_STEPS steps, each
-    // with its own event pairs around the pair kernel; replays cycle through
-    // them and the events are read after every TIMING_RING replays, before
-    // any graph is replayed (and its events recorded) again
-    struct TGraph { hipGraph_t g = nullptr; hipGraphExec_t ge = nullptr; std::vector<TimedLaunch> ev; };
-    std::vector<TGraph> tring;
-    int tring_next = 0, tring_unread = 0;
 };
 
 static hipEvent_t take_event(mfa_ctx *c) {
@@ -127,10 +115,7 @@ struct Timer {
     ~Timer() {
         if (a) {
             hipEvent_t b = take_event(c);
-            if (b) {
-                (void)hipEventRecord(b, c->st);
-                (c->sink ? c->sink : &c->pending)->push_back({a, b, kid});
-            }
+            if (b) { (void)hipEventRecord(b, c->st); c->pending.push_back({a, b, kid}); }
         }
     }
 };
@@ -144,7 +129,7 @@ static void add_elapsed(mfa_ctx *c, const TimedLaunch &t) {
 }
 
 static int collect_timing(mfa_ctx *c) {
-    if (c->pending.empty() && c->tring_unread == 0) return 0;
+    if (c->pending.empty()) return 0;
     HIPCHK(hipStreamSynchronize(c->st));
     for (auto &t : c->pending) {
         add_elapsed(c, t);
@@ -152,13 +137,6 @@ static int collect_timing(mfa_ctx *c) {
         c->ev_pool.push_back(t.b);
     }
     c->pending.clear();
-    // the timing graphs replayed since the last read, oldest first
-    const int R = (int)c->tring.size();
-    for (int q = 0; q < c->tring_unread; ++q) {
-        const int g = ((c->tring_next - c->tring_unread + q) % R + R) % R;
-        for (auto &t : c->tring[g].ev) add_elapsed(c, t);
-    }
-    c->tring_unread = 0;
     return 0;
 }
 
@@ -223,12 +201,6 @@ static void free_all(mfa_ctx *c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->gexec_k) (void)hipGraphExecDestroy(c->gexec_k);
     if (c->graph_k) (void)hipGraphDestroy(c->graph_k);
-    for (auto &tg : c->tring) {
-        if (tg.ge) (void)hipGraphExecDestroy(tg.ge);
-        if (tg.g) (void)hipGraphDestroy(tg.g);
-        for (auto &t : tg.ev) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
-    }
-    c->tring.clear();
     if (c->st) (void)hipStreamDestroy(c->st);
 }
 
@@ -471,20 +443,11 @@ static int check_ctx(mfa_ctx *c, bool need_state = true) {
     return 0;
 }
 
-// (callers synchronise the stream first, so no replay is in flight; the
-// timing graphs' events return to the pool after their last read)
 static void invalidate_graph(mfa_ctx *c) {
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     if (c->gexec_k) { (void)hipGraphExecDestroy(c->gexec_k); c->gexec_k = nullptr; }
     if (c->graph_k) { (void)hipGraphDestroy(c->graph_k); c->graph_k = nullptr; }
-    for (auto &tg : c->tring) {
-        if (tg.ge) (void)hipGraphExecDestroy(tg.ge);
-        if (tg.g) (void)hipGraphDestroy(tg.g);
-        for (auto &t : tg.ev) { c->ev_pool.push_back(t.a); c->ev_pool.push_back(t.b); }
-    }
-    c->tring.clear();
-    c->tring_next = c->tring_unread = 0;
 }
 
 // Captures `steps` consecutive steps into one executable graph.  Every step
@@ -797,46 +760,12 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
     if (n_steps < 0) return fail(MFA_E_ARG, "md_steps: negative step count");
     ensure_cells(c);
     if (n_steps > 0) hipLaunchKernelGGL(k_set_t_end, dim3(1), dim3(1), 0, c->st, c->B, (int)n_steps);
-    if (!use_graph || (c->timing_mask && !c->timing_graphs)) {
+    if (!use_graph || c->timing_mask) {
         for (int i = 0; i < n_steps; ++i) enqueue_step(c);
         HIPCHK(hipGetLastError());
         return 0;
     }
     int i = 0;
-    if (c->timing_graphs) {
-        // pair-only timing: replay the ring of timing graphs, read their
-        // events after every full turn; the remainder runs eagerly, timed
-        if (c->tring.empty()) {
-            c->tring.resize(TIMING_RING);
-            int rc = 0;
-            for (int g = 0; g < TIMING_RING && !rc; ++g) {
-                c->sink = &c->tring[g].ev;
-                rc = capture_steps(c, GRAPH_STEPS, &c->tring[g].g, &c->tring[g].ge);
-                c->sink = nullptr;
-            }
-            if (rc) {
-                // event records not capturable here: time the pair kernel
-                // with eager launches instead (capture_steps dropped the ring)
-                (void)hipGetLastError();
-                c->timing_graphs = false;
-            }
-        }
-        if (!c->timing_graphs) {
-            for (int k = 0; k < n_steps; ++k) enqueue_step(c);
-            HIPCHK(hipGetLastError());
-            return 0;
-        }
-        for (; i + GRAPH_STEPS <= n_steps; i += GRAPH_STEPS) {
-            if (c->tring_unread == TIMING_RING)
-                if (int rc = collect_timing(c)) return rc;
-            HIPCHK(hipGraphLaunch(c->tring[c->tring_next].ge, c->st));
-            c->tring_next = (c->tring_next + 1) % TIMING_RING;
-            c->tring_unread += 1;
-        }
-        for (; i < n_steps; ++i) enqueue_step(c);
-        HIPCHK(hipGetLastError());
-        return 0;
-    }
     if (n_steps >= GRAPH_STEPS) {
         if (!c->gexec_k)
             if (int rc = capture_steps(c, GRAPH_STEPS, &c->graph_k, &c->gexec_k)) return rc;
@@ -850,21 +779,8 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
 
 int mfa_hip_prepare_graphs(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
+    if (c->timing_mask) return 0;       // timed steps launch eagerly
     HIPCHK(hipStreamSynchronize(c->st));
-    if (c->timing_graphs) {
-        if (c->tring.empty()) {
-            c->tring.resize(TIMING_RING);
-            int rc = 0;
-            for (int g = 0; g < TIMING_RING && !rc; ++g) {
-                c->sink = &c->tring[g].ev;
-                rc = capture_steps(c, GRAPH_STEPS, &c->tring[g].g, &c->tring[g].ge);
-                c->sink = nullptr;
-            }
-            if (rc) { (void)hipGetLastError(); c->timing_graphs = false; }
-        }
-        return 0;
-    }
-    if (c->timing_mask) return 0;       // eager launches while every group is timed
     if (!c->gexec_k)
         if (int rc = capture_steps(c, GRAPH_STEPS, &c->graph_k, &c->gexec_k)) return rc;
     if (!c->gexec)
@@ -1085,10 +1001,7 @@ int mfa_hip_set_timing(mfa_ctx *c, int32_t on) {
     if (int rc = check_ctx(c, false)) return rc;
     if (on < 0 || on > 2) return fail(MFA_E_ARG, "set_timing: mode must be 0, 1 or 2");
     if (int rc = collect_timing(c)) return rc;
-    const bool graphs = on == 2;
-    if (graphs != c->timing_graphs || (on != 0) != (c->timing_mask != 0)) invalidate_graph(c);
     c->timing_mask = on == 1 ? (1 << K_NK) - 1 : on == 2 ? (1 << K_PAIR) : 0;
-    c->timing_graphs = graphs;
     for (int k = 0; k < K_NK; ++k) { c->k_ms[k] = 0.0; c->k_n[k] = 0; }
     return 0;
 }

@@ -110,10 +110,9 @@ def test_full_size_force_error(oracle_mod, workload):
             assert err_max < 1e-5 and err_rms < 2e-6 and err_e < 1e-5
 
 
-def test_pair_timing_graphs_same_bits():
-    """Pair-kernel timing with the ring of timing graphs (event records as
-    graph nodes) times every pair launch and leaves the trajectory's bits
-    unchanged."""
+def test_pair_timing_same_bits():
+    """Pair-kernel timing (events around each pair launch, eager launches)
+    times every pair launch and leaves the trajectory's bits unchanged."""
     p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
     p = p.replace(skin=0.4)
     outs = []
@@ -122,7 +121,7 @@ def test_pair_timing_graphs_same_bits():
             s.set_state(r0, v, t)
             s.set_timing(mode)
             s.prepare_graphs()
-            s.md_steps(45, graph=True)          # 5 graph replays (one more than the ring) + 5 eager steps
+            s.md_steps(45, graph=True)
             s.synchronize()
             ms, n = s.kernel_time("pair")
             outs.append(s.get_state())
