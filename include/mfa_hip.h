@@ -146,6 +146,25 @@ int  mfa_hip_get_f_on_heads(mfa_ctx *ctx, int32_t isys, double *f_on_heads /* (3
 int  mfa_hip_get_flags(mfa_ctx *ctx, mfa_flags *out);
 /* check_fluid_fluid, util.f90:284-330: O(N^2) brute-force LJ energy */
 int  mfa_hip_check_fluid_fluid(mfa_ctx *ctx, int32_t isys, double *v_fluid_fluid);
+/* check_fluid_fluid without any allocation.  No counterpart in the reference:
+ * an extra entry point for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same result bits as mfa_hip_check_fluid_fluid on the same positions, but
+ * on caller-owned device arrays and without a context:
+ *   d_r0      device, (3, n_part) FP64 in the commons layout
+ *   d_a_type  device, n_part int32 in 1..4
+ *   r_2_min   switch-on clamp of r^2 (0 when FORCE_SWITCH_ON is off)
+ *   d_v_ff    device, one double: receives V_ff
+ *   stream    a hipStream_t, or NULL for the null stream
+ * Only sys->boundary, epsil, sigma_2, range_2 and e_shift are read.  The
+ * per-bead energies go to a static device array of MFA_NOALLOC_MAX_PART
+ * doubles (8 MiB, held by each GPU the library runs on until the process
+ * ends); n_part beyond it is rejected with MFA_E_ARG.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race.  The call is asynchronous on `stream`. */
+#define MFA_NOALLOC_MAX_PART (1 << 20)
+int  mfa_hip_check_fluid_fluid_noalloc(const mfa_system_params *sys, const double *d_r0, const int32_t *d_a_type,
+                                       int32_t n_part, double r_2_min, double *d_v_ff, void *stream);
 /* interacting-pair fingerprint per bead (reference order): number of partners
  * with r^2 < range_2 and the wrapping sum of mix64(partner index) */
 int  mfa_hip_pair_stats(mfa_ctx *ctx, int32_t isys, int32_t *count, uint64_t *hash);

@@ -79,6 +79,7 @@ SIGNATURES = {
     "mfa_hip_get_flags": [_vp, _vp],
     "mfa_hip_check_fluid_fluid": [_vp, _i32, _vp],
     "mfa_hip_pair_stats": [_vp, _i32, _vp, _vp],
+    "mfa_hip_check_fluid_fluid_noalloc": [_vp, _vp, _vp, _i32, _d, _vp, _vp],
     "mfa_hip_profiles_reset": [_vp, _i32],
     "mfa_hip_profiles_accumulate": [_vp],
     "mfa_hip_profiles_get": [_vp, _i32, _vp, _vp, _vp, _vp],

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -466,6 +467,107 @@ static int capture_steps(mfa_ctx *c, int steps, hipGraph_t *g, hipGraphExec_t *g
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Allocation-free check_fluid_fluid (mfa_hip_check_fluid_fluid_noalloc).  The
+// per-bead energies live in `scratch`, a file-scope device array each GPU
+// holds for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int NOALLOC_MAX_PART = MFA_NOALLOC_MAX_PART;
+constexpr int MAX_DEVICES = 64;
+
+static __device__ __align__(256) double scratch[NOALLOC_MAX_PART];
+
+struct BruteTables {
+    double L0, L1, iL0, iL1;
+    double epsil[16], sigma_2[16], range_2[16], e_shift[16];
+};
+
+// Per-bead part of util.f90:284-330 over caller arrays in the reference
+// layout: e_i = sum over j > i of the LJ energy, j ascending, with the same
+// expressions as k_brute, both without FP contraction (so the energies are
+// the same bits, and the reference's x86-64 arithmetic).
+__global__ __launch_bounds__(BLK) void k_brute_flat(BruteTables T, const double *__restrict__ r0,
+                                                   const int *__restrict__ a_type, int n, double r_2_min,
+                                                   double *__restrict__ e_out) {
+#pragma clang fp contract(off)
+    __shared__ double4 tile[BLK];
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    const bool act = i < n;
+    double4 ri = make_double4(0.0, 0.0, 0.0, 0.0);
+    int ti = 1;
+    if (act) {
+        ri = make_double4(r0[3 * (size_t)i], r0[3 * (size_t)i + 1], r0[3 * (size_t)i + 2], 0.0);
+        ti = a_type[i];
+    }
+    double e = 0.0;
+    for (int j0 = 0; j0 < n; j0 += BLK) {
+        const int jj = j0 + threadIdx.x;
+        tile[threadIdx.x] = jj < n ? make_double4(r0[3 * (size_t)jj], r0[3 * (size_t)jj + 1], r0[3 * (size_t)jj + 2],
+                                                  __longlong_as_double((long long)a_type[jj]))
+                                   : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const int jn = min(BLK, n - j0);
+        if (act) {
+            for (int q = 0; q < jn; ++q) {
+                const int j = j0 + q;
+                if (j <= i) continue;
+                const double4 rj = tile[q];
+                const int tj = (int)__double_as_longlong(rj.w);
+                const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
+                const double dx = mic(ri.x - rj.x, T.L0, T.iL0);
+                const double dy = mic(ri.y - rj.y, T.L1, T.iL1);
+                const double dz = ri.z - rj.z;
+                double r_2 = r2_exact(dx, dy, dz);
+                if (r_2 < T.range_2[tt]) {
+                    if (r_2 < r_2_min) r_2 = r_2_min;
+                    const double qd = T.sigma_2[tt] / r_2;
+                    const double r_6 = qd * qd * qd;
+                    const double r_12 = r_6 * r_6;
+                    e += T.epsil[tt] * ((r_12 - r_6) - T.e_shift[tt]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (act) e_out[i] = e;
+}
+
+// v = sum of e[0..n-1] in index order (one thread: the order of the host sum
+// of mfa_hip_check_fluid_fluid, so the result is the same bits).
+__global__ void k_sum_in_order(const double *__restrict__ e, int n, double *__restrict__ v) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += e[i];
+    *v = s;
+}
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of this device's `scratch`, looked up once per device with
+// that device current.
+int scratch_for_current_device(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(MFA_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(MFA_E_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MFA_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *mfa_hip_last_error(void) { return g_err.c_str(); }
@@ -906,6 +1008,32 @@ int mfa_hip_check_fluid_fluid(mfa_ctx *c, int32_t isys, double *v) {
     if (int rc = check_ctx(c)) return rc;
     if (isys < 0 || isys >= c->n_sys || !v) return fail(MFA_E_ARG, "check_fluid_fluid: bad argument");
     return brute(c, isys, v, nullptr, nullptr);
+}
+
+int mfa_hip_check_fluid_fluid_noalloc(const mfa_system_params *sys, const double *d_r0, const int32_t *d_a_type,
+                                      int32_t n_part, double r_2_min, double *d_v_ff, void *stream) {
+    if (!sys || !d_r0 || !d_a_type || !d_v_ff) return fail(MFA_E_ARG, "check_fluid_fluid_noalloc: null argument");
+    if (n_part < 1) return fail(MFA_E_ARG, "check_fluid_fluid_noalloc: n_part must be >= 1");
+    if (n_part > NOALLOC_MAX_PART)
+        return fail(MFA_E_ARG, "check_fluid_fluid_noalloc: n_part " + std::to_string(n_part) +
+                                   " exceeds the scratch capacity of " + std::to_string(NOALLOC_MAX_PART) + " beads");
+    if (!(sys->boundary[0] > 0.0) || !(sys->boundary[1] > 0.0))
+        return fail(MFA_E_ARG, "check_fluid_fluid_noalloc: boundary must be positive");
+    double *scr = nullptr;
+    if (int rc = scratch_for_current_device(&scr)) return rc;
+    BruteTables T;
+    T.L0 = sys->boundary[0]; T.L1 = sys->boundary[1];
+    T.iL0 = 1.0 / sys->boundary[0]; T.iL1 = 1.0 / sys->boundary[1];
+    for (int t = 0; t < 16; ++t) {
+        T.epsil[t] = sys->epsil[t]; T.sigma_2[t] = sys->sigma_2[t];
+        T.range_2[t] = sys->range_2[t]; T.e_shift[t] = sys->e_shift[t];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_brute_flat, dim3((n_part + BLK - 1) / BLK), dim3(BLK), 0, st, T, d_r0, (const int *)d_a_type,
+                       (int)n_part, r_2_min, scr);
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, st, (const double *)scr, (int)n_part, d_v_ff);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int mfa_hip_pair_stats(mfa_ctx *c, int32_t isys, int32_t *count, uint64_t *hash) {

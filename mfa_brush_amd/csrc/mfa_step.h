@@ -590,6 +590,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 __global__ __launch_bounds__(BLK) void k_brute(const SysDev *__restrict__ sys, int s, Bufs B, double r_2_min,
                                               double *__restrict__ e_out, int *__restrict__ cnt_out,
                                               unsigned long long *__restrict__ hash_out) {
+#pragma clang fp contract(off)
     MFA_CUR(B, s);
     __shared__ double4 tile[BLK];
     const SysDev &P = sys[s];
