@@ -107,7 +107,7 @@ def build_workload(name, skin, n_rep=256):
     if name == "c3":
         p, t, r0, v = configs.channel_melt(thermostat=True, rc_dpd=1.0, skin=skin)
         return [p], [(t, r0, v)], ("melt channel 1,200,000 beads, 18,750 chains N=64, box 216.93x216.93x30, "
-                                   "FENE + harmonic bending k_b 5, Poiseuille g 0.01 x, DPD rc 1, dt 0.005")
+                                   "FENE + cosine bending k_b 5, Poiseuille g 0.01 x, DPD rc 1, dt 0.005")
     if name == "c4":
         from concurrent.futures import ProcessPoolExecutor
         with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:

@@ -46,7 +46,8 @@ typedef struct mfa_system_params {
     int32_t bending_melt;         /* BENDING_MELT (control_simulation.h:48) */
     int32_t poiseuille;           /* POISEUILLE (control_simulation.h:80) */
     int32_t l_box;                /* L_BOX 1 or 2 (control_simulation.h:105) */
-    int32_t reserved0;
+    int32_t bend_form;            /* bending terms: 0 the reference's harmonic angle 0.5 k_bend (theta - alpha_eq)^2
+                                     (bending.f90:97-100); 1 cosine k_bend (1 - cos theta) (no counterpart) */
     double boundary[3];           /* box; z = z_space_wall (init_config.f90:17-19) */
     double epsil[16];             /* 4*epsilon, epsil(1,1)=0 */
     double sigma_2[16];
@@ -72,6 +73,9 @@ typedef struct mfa_system_params {
     double mass_type[4];
     uint64_t seed;                /* Philox key of the DPD noise */
     int64_t t_idx0;               /* time index of the next step: r_time = t_idx*dt */
+    int32_t dpd_vv;               /* DPD_VV (control_simulation.h:65): md_steps re-evaluates the dissipative
+                                     forces after the second half kick (new_dpd_fd.f90) */
+    int32_t reserved1;
 } mfa_system_params;
 
 typedef struct mfa_energies {
@@ -128,6 +132,16 @@ int  mfa_hip_bending(mfa_ctx *ctx, int32_t mode);  /* bending.f90:19-121 (mode 1
 int  mfa_hip_bending_melt(mfa_ctx *ctx, int32_t mode);   /* bending_melt.f90:22-120 (mode 1) */
 int  mfa_hip_constant_force(mfa_ctx *ctx, int32_t case_pois); /* constant_force.f90:17-21 (case 1) */
 int  mfa_hip_verlet_velocities(mfa_ctx *ctx);      /* verlet_velocities.f90:18-153; ends the step */
+/* new_dpd_fd.f90:1-216 (md_main.f90:187-198, DPD_VV), right after
+ * verlet_velocities: the dissipative forces of every DPD channel are
+ * evaluated again at the new velocities and replace those of the half-step
+ * velocities in the force and the acceleration the next step starts from.
+ * The velocities and the kinetic energy are unchanged.  (The reference's
+ * routine also subtracts a force_d that its pair routine never fills and
+ * adds a further half kick; this follows the published DPD-VV scheme,
+ * Besold et al., PRE 62, R7611 (2000), which only replaces F_D.)  Heads keep
+ * zero force; a head's velocity change is taken as zero (heads are frozen). */
+int  mfa_hip_new_dpd_fd(mfa_ctx *ctx);
 
 /* Fused steps: n_steps of md_main.f90:77-185 in 4 kernels per step
  * (integrate+skin check, [gated cell rebuild], pair, bonded+wall+kick).

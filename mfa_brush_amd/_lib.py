@@ -26,7 +26,7 @@ class SystemParams(C.Structure):
         ("n_mon", C.c_int32), ("n_chain", C.c_int32), ("n_mon_d", C.c_int32), ("n_chain_d", C.c_int32),
         ("thermostat", C.c_int32), ("dpd_cut_off", C.c_int32), ("force_switch_on", C.c_int32),
         ("bending", C.c_int32), ("bending_melt", C.c_int32), ("poiseuille", C.c_int32),
-        ("l_box", C.c_int32), ("reserved0", C.c_int32),
+        ("l_box", C.c_int32), ("bend_form", C.c_int32),
         ("boundary", _D3),
         ("epsil", _D16), ("sigma_2", _D16), ("range_2", _D16), ("e_shift", _D16), ("inv_range_2", _D16),
         ("r_cut_max", C.c_double), ("r_cut_dpd_2", C.c_double), ("skin", C.c_double),
@@ -37,7 +37,7 @@ class SystemParams(C.Structure):
         ("k_bend", C.c_double), ("alpha_eq", C.c_double),
         ("dt", C.c_double),
         ("const_force", _D3), ("head_vel", _D2), ("mass_type", _D4),
-        ("seed", C.c_uint64), ("t_idx0", C.c_int64),
+        ("seed", C.c_uint64), ("t_idx0", C.c_int64), ("dpd_vv", C.c_int32), ("reserved1", C.c_int32),
     ]
 
 
@@ -71,6 +71,7 @@ SIGNATURES = {
     "mfa_hip_bending_melt": [_vp, _i32],
     "mfa_hip_constant_force": [_vp, _i32],
     "mfa_hip_verlet_velocities": [_vp],
+    "mfa_hip_new_dpd_fd": [_vp],
     "mfa_hip_md_steps": [_vp, _i32, _i32],
     "mfa_hip_prepare_graphs": [_vp],
     "mfa_hip_synchronize": [_vp],

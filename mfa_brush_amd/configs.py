@@ -362,15 +362,16 @@ def channel_general(n_side: int, a_graft: float, n_mon: int, lz: float, density:
 
 
 def channel_melt(n_cols: int = 200, n_z: int = 30, n_mon: int = 64, lz: float = 30.0, density: float = 0.85,
-                 k_bend: float = 5.0, g: float = 0.01, seed: int = 3, **kw):
+                 k_bend: float = 5.0, g: float = 0.01, seed: int = 3, bend_cosine: bool = True, **kw):
     """Melt in a channel (no brush): n_cols^2 columns of n_z sites fill the
     box at `density`; chains of n_mon beads follow a serpentine path through
     the columns (up one column, down the next, rows of columns in
     alternating x direction), so every bond is either the in-column site
-    spacing or the column spacing.  FENE + harmonic-angle bending
-    (bending_melt, k_bend) and a Poiseuille body force g along x on every
-    bead.  The default is the 1.2M-bead melt (18,750 chains of 64,
-    L = 216.93, Lz = 30)."""
+    spacing or the column spacing.  FENE + bending of the melt chains
+    (bending_melt, k_bend): the cosine form k_bend (1 - cos theta) by
+    default, the reference's harmonic angle with bend_cosine=False; a
+    Poiseuille body force g along x on every bead.  The default is the
+    1.2M-bead melt (18,750 chains of 64, L = 216.93, Lz = 30)."""
     n_sites = n_cols * n_cols * n_z
     if n_sites % n_mon:
         raise ValueError("columns x sites must be a multiple of the chain length")
@@ -383,7 +384,8 @@ def channel_melt(n_cols: int = 200, n_z: int = 30, n_mon: int = 64, lz: float = 
     p = channel_params((L, L, lz), 0, 0, n_sites // n_mon, bending=False, k_bend=k_bend,
                        const_force=(g, 0.0, 0.0), **kw)
     p = p.replace(n_mon_d=n_mon, n_chain_d=n_sites // n_mon,
-                  flags=Flags(**{**p.flags.__dict__, "bending_melt": k_bend > 0.0, "poiseuille": g != 0.0}))
+                  flags=Flags(**{**p.flags.__dict__, "bending_melt": k_bend > 0.0, "poiseuille": g != 0.0,
+                                 "bend_cosine": bend_cosine}))
     r0 = np.zeros((n_sites, 3))
     k = 0
     col = 0

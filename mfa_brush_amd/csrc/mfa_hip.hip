@@ -50,8 +50,8 @@ struct TimedLaunch {
     int kernel;
 };
 
-enum KId { K_VV1 = 0, K_REBUILD, K_PAIR, K_BONDED, K_NK };
-const char *kNames[K_NK] = {"vv1", "rebuild", "pair", "bonded"};
+enum KId { K_VV1 = 0, K_REBUILD, K_PAIR, K_BONDED, K_DPDVV, K_NK };
+const char *kNames[K_NK] = {"vv1", "rebuild", "pair", "bonded", "dpd_vv"};
 
 }  // namespace
 
@@ -68,6 +68,7 @@ struct mfa_ctx {
     int2 *d_crange = nullptr;
     int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_key_tmp = nullptr, *d_bin_key = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
+    double4 *d_dfd = nullptr;      // DPD-VV: change of each bead's dissipative force (slot order)
     int *d_rebuild_count = nullptr, *d_permute_done = nullptr;
     double *d_obs = nullptr;       // block partials of the chain observables (2 x channel blocks x 8)
     double *d_stage = nullptr;
@@ -86,8 +87,8 @@ struct mfa_ctx {
     int timing_mask = 0;
     std::vector<hipEvent_t> ev_pool;
     std::vector<TimedLaunch> pending;
-    double k_ms[K_NK] = {0, 0, 0, 0};
-    long long k_n[K_NK] = {0, 0, 0, 0};
+    double k_ms[K_NK] = {};
+    long long k_n[K_NK] = {};
     // graph
     hipGraphExec_t gexec = nullptr;      // one step
     hipGraph_t graph = nullptr;
@@ -194,7 +195,7 @@ static void free_all(mfa_ctx *c) {
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_obs); F(c->d_stage); F(c->d_istage);
-    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2);
+    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2); F(c->d_dfd);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -290,6 +291,9 @@ static int build_sysdev(const mfa_system_params &p, SysDev &d) {
     d.pd = p.n_mon * p.n_chain;
     d.thermostat = p.thermostat; d.dpd_cut_off = p.dpd_cut_off; d.force_switch_on = p.force_switch_on;
     d.bending = p.bending; d.bending_melt = p.bending_melt; d.poiseuille = p.poiseuille;
+    if (p.bend_form != 0 && p.bend_form != 1) return fail(MFA_E_ARG, "bend_form must be 0 (harmonic) or 1 (cosine)");
+    d.bend_form = p.bend_form;
+    d.dpd_vv = p.dpd_vv != 0;
     if (d.n_part < 1) return fail(MFA_E_ARG, "channel without beads");
     return 0;
 }
@@ -408,14 +412,26 @@ static void enqueue_bonded(mfa_ctx *c) {
         LAUNCH((k_bonded<MODE, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
 }
 
+// DPD-VV re-evaluation after the second half kick (new_dpd_fd.f90); all:
+// every DPD channel, else those with dpd_vv set; fused: inside md_steps
+static void enqueue_dpd_vv(mfa_ctx *c, int all, int fused) {
+    Timer t(c, K_DPDVV);
+    LAUNCH(k_dpd_vv, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const int2 *)c->d_crange, c->d_dfd, all);
+    LAUNCH(k_dpd_vv_apply, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const double4 *)c->d_dfd, all, fused);
+}
+
 static void enqueue_step(mfa_ctx *c) {
     enqueue_vv1(c);
     enqueue_rebuild(c, 0);
     enqueue_pair(c);
-    bool bend = false;
-    for (const SysDev &d : c->hs) bend = bend || d.bending || d.bending_melt;
+    bool bend = false, dpd_vv = false;
+    for (const SysDev &d : c->hs) {
+        bend = bend || d.bending || d.bending_melt;
+        dpd_vv = dpd_vv || (d.dpd_vv && d.thermostat == 1);
+    }
     if (bend) enqueue_bonded<M_FUSED>(c);
     else enqueue_bonded<M_FUSED | M_NOBEND>(c);
+    if (dpd_vv) enqueue_dpd_vv(c, 0, 1);
 }
 
 // Reads the device guard word (one sync): a nonzero code means a kernel met
@@ -664,6 +680,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
         ALLOC(c->d_obs, 16 * (size_t)mb);
     }
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
+    ALLOC(c->d_dfd, c->S_tot);
 #undef ALLOC
     if ((rc = alloc_list(c))) { free_all(c); delete c; return rc; }
     // the zero fills of dalloc ran on the null stream: finish them before
@@ -853,6 +870,13 @@ int mfa_hip_constant_force(mfa_ctx *c, int32_t case_pois) {
 int mfa_hip_verlet_velocities(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     enqueue_bonded<M_VV2>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mfa_hip_new_dpd_fd(mfa_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    enqueue_dpd_vv(c, 1, 0);
     HIPCHK(hipGetLastError());
     return 0;
 }

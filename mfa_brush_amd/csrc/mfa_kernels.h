@@ -39,7 +39,7 @@ struct SysDev {
     int nc[3], n_cells_tot, cell_base;
     int slot_base, n_part, part_base;
     int n_mon, n_chain, n_mon_d, n_chain_d, pd, head_base;
-    int thermostat, dpd_cut_off, force_switch_on, bending, bending_melt, poiseuille;
+    int thermostat, dpd_cut_off, force_switch_on, bending, bending_melt, poiseuille, bend_form, dpd_vv;
     int S, skin_mode;
     int block_base, n_blocks;
 };

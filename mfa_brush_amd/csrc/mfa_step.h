@@ -103,7 +103,8 @@ __device__ __forceinline__ BondOut fene(const SysDev &P, const double4 &ra, cons
 struct BendOut { double fp[3], fn[3], v; };
 
 // Harmonic-angle bending centred on c, bending.f90:30-109 (including the
-// small-angle fallback :59-92 and the cos >= 1 guard :93-98).
+// small-angle fallback :59-92 and the cos >= 1 guard :93-98); with
+// bend_form 1 the cosine form k_bend (1 - cos theta) over the same angle.
 __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm, const double4 &rc, const double4 &rpn) {
 #pragma clang fp contract(off)
     BendOut o;
@@ -117,6 +118,16 @@ __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm
     double dot = 0.0, rp2 = 0.0, rn2 = 0.0;
     for (int d = 0; d < 3; ++d) { dot += rn[d] * rp[d]; rp2 += rp[d] * rp[d]; rn2 += rn[d] * rn[d]; }
     double cosa = dot / sqrt(rp2 * rn2);
+    if (P.bend_form == 1) {
+        // cosine form k (1 - cos theta): fp = k dcos/dr_prev, fn = k dcos/dr_next
+        const double inv_pn = 1.0 / sqrt(rp2 * rn2);
+        o.v = P.k_bend * (1.0 - cosa);
+        for (int d = 0; d < 3; ++d) {
+            o.fp[d] = P.k_bend * (cosa * rp[d] / rp2 - rn[d] * inv_pn);
+            o.fn[d] = P.k_bend * (rp[d] * inv_pn - cosa * rn[d] / rn2);
+        }
+        return o;
+    }
     double dn[3], dp[3], dn2 = 0.0, dp2 = 0.0;
     for (int d = 0; d < 3; ++d) { dn[d] = rp[d] * rn2 - rn[d] * dot; dp[d] = -rn[d] * rp2 + rp[d] * dot; }
     for (int d = 0; d < 3; ++d) { dp2 += dp[d] * dp[d]; dn2 += dn[d] * dn[d]; }
@@ -307,6 +318,125 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         double t = block_sum(ke, red);
         if (threadIdx.x == 0) out[C_KE] = t;
         if (blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DPD-VV (new_dpd_fd.f90, md_main.f90:187-198): after the second half kick
+// the dissipative forces are evaluated again at the new velocities and
+// replace those of the half-step velocities in the force and acceleration
+// carried into the next step.  The dissipative force is linear in the
+// velocity differences, so F_D(v_new) - F_D(v_half) = F_D(v_new - v_half),
+// and v_new - v_half is the kick just applied, dt/2 a (zero for the heads,
+// whose velocities are fixed).  k_dpd_vv sums that difference per bead over
+// the bead's DPD pairs (the pair set and weights of the pair kernel's DPD
+// part: with DPD_CUT_OFF the non-head pairs within rc_dpd, else the pairs
+// within the LJ range, with the switch-on clamp) into dfd; k_dpd_vv_apply
+// adds it to the acceleration and the force.  `all`: every DPD channel (the
+// routine-level call), else only the channels with dpd_vv set.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool dpd_vv_on(const SysDev &P, int all) {
+    return P.thermostat == 1 && (all || P.dpd_vv);
+}
+
+__global__ __launch_bounds__(BLK) void k_dpd_vv(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
+                                               const int2 *__restrict__ crange, double4 *__restrict__ dfd, int all) {
+#pragma clang fp contract(off)
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k - P.slot_base >= P.n_part || !dpd_vv_on(P, all)) return;
+    const double4 *__restrict__ pos = CUR(B.pos);
+    const double4 *__restrict__ acc = CUR(B.acc);
+    // the step's pair evaluation ran at the time index before the kick advanced it
+    double r_2_min = 0.0;
+    if (P.force_switch_on) {
+        const double r_time = (double)(*B.t_idx - 1) * P.dt;
+        if (r_time < P.r_2_min_time) r_2_min = (1.0 - r_time / P.r_2_min_time) * P.r_2_min_init;
+    }
+    const double hd = 0.5 * P.dt;
+    const double4 pi = pos[k];
+    const int ti = wtype(pi.w);
+    const double4 ai = acc[k];
+    const double dvix = hd * ai.x, dviy = hd * ai.y, dviz = hd * ai.z;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    auto visit = [&](int m) {
+        const double4 pj = pos[m];
+        const int tj = wtype(pj.w);
+        const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
+        const double dx = mic(pi.x - pj.x, P.L[0], P.invL[0]);
+        const double dy = mic(pi.y - pj.y, P.L[1], P.invL[1]);
+        const double dz = pi.z - pj.z;
+        double r_2 = r2_exact(dx, dy, dz);
+        const bool lj = r_2 < P.range_2[tt];
+        const bool dpd = P.dpd_cut_off ? (ti != 1 && tj != 1 && r_2 < P.r_cut_dpd_2) : lj;
+        if (!dpd) return;
+        if (lj && P.force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+        const double inv_sqrt = sqrt(1.0 / r_2);
+        const double w = P.dpd_cut_off ? 1.0 - sqrt(r_2 / P.r_cut_dpd_2) : 1.0 - sqrt(r_2 * P.inv_range_2[tt]);
+        const double4 aj = acc[m];
+        const double rvx = dx * inv_sqrt, rvy = dy * inv_sqrt, rvz = dz * inv_sqrt;
+        const double rdv = (dvix - hd * aj.x) * rvx + (dviy - hd * aj.y) * rvy + (dviz - hd * aj.z) * rvz;
+        const double sd = -1.0 * P.friction * (w * w) * rdv;
+        fx += sd * rvx; fy += sd * rvy; fz += sd * rvz;
+    };
+    const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
+    if (nn >= 0 && nn <= B.nl_cap) {
+        for (int q = 0; q < nn; ++q) {
+            const unsigned m = (unsigned)B.nl[nl_index(k, q, (size_t)B.n_slots)] >> NL_CODE_BITS;
+            if (guard(B, m < (unsigned)B.n_slots, G_SLOT, k)) visit((int)m);
+        }
+    } else {
+        int c = CUR(B.cell_of)[k] - P.cell_base;
+        if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
+        const int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        for (int dz = -P.S; dz <= P.S; ++dz) {
+            const int z2 = cz + dz;
+            if (z2 < 0 || z2 >= nz) continue;
+            for (int dy = -P.S; dy <= P.S; ++dy) {
+                int y2 = cy + dy;
+                y2 += (y2 < 0) ? ny : 0;
+                y2 -= (y2 >= ny) ? ny : 0;
+                const int row = P.cell_base + nx * (y2 + ny * z2);
+                for (int dx = -P.S; dx <= P.S; ++dx) {
+                    int x2 = cx + dx;
+                    x2 += (x2 < 0) ? nx : 0;
+                    x2 -= (x2 >= nx) ? nx : 0;
+                    int2 rg = crange[row + x2];
+                    if (!guard(B, rg.x >= 0 && rg.x <= rg.y && rg.y <= B.n_slots, G_SLOT, k)) rg = make_int2(0, 0);
+                    for (int m = rg.x; m < rg.y; ++m)
+                        if (m != k) visit(m);
+                }
+            }
+        }
+    }
+    dfd[k] = make_double4(fx, fy, fz, 0.0);
+}
+
+// fused: the fused step, whose force is stored only on a call's last step
+__global__ __launch_bounds__(BLK) void k_dpd_vv_apply(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+                                                     Bufs B, const double4 *__restrict__ dfd, int all, int fused) {
+#pragma clang fp contract(off)
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k - P.slot_base >= P.n_part || !dpd_vv_on(P, all)) return;
+    const double4 ri = CUR(B.pos)[k];
+    const int ti = wtype(ri.w);
+    const int o = worig(ri.w) - P.part_base;
+    if (o < P.pd && P.n_mon > 0 && o % P.n_mon == 0) return;     // heads keep force and acceleration zero
+    const double4 d = dfd[k];
+    const double im = P.inv_mass_type[(ti - 1) & 3];
+    double4 a = CUR(B.acc)[k];
+    a.x = a.x + d.x * im; a.y = a.y + d.y * im; a.z = a.z + d.z * im;
+    CUR(B.acc)[k] = a;
+    if (!fused || *B.store_frc) {
+        double4 f = CUR(B.frc)[k];
+        f.x = f.x + d.x; f.y = f.y + d.y; f.z = f.z + d.z;
+        CUR(B.frc)[k] = f;
     }
 }
 

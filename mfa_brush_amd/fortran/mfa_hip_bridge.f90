@@ -26,13 +26,14 @@ module mfa_hip_iface
     type, bind(C) :: mfa_system_params
         integer(c_int32_t) :: n_mon, n_chain, n_mon_d, n_chain_d
         integer(c_int32_t) :: thermostat, dpd_cut_off, force_switch_on, bending, bending_melt, poiseuille
-        integer(c_int32_t) :: l_box, reserved0
+        integer(c_int32_t) :: l_box, bend_form
         real(c_double) :: boundary(3)
         real(c_double) :: epsil(16), sigma_2(16), range_2(16), e_shift(16), inv_range_2(16)
         real(c_double) :: r_cut_max, r_cut_dpd_2, skin, sig, friction, r_2_min_init, r_2_min_time
         real(c_double) :: sigma_wall(4), a_wall(4), a_w, k_chain, inv_r_chain_2, k_bend, alpha_eq, dt
         real(c_double) :: const_force(3), head_vel(2), mass_type(4)
         integer(c_int64_t) :: seed, t_idx0
+        integer(c_int32_t) :: dpd_vv, reserved1
     end type
 
     type, bind(C) :: mfa_energies
@@ -99,6 +100,10 @@ module mfa_hip_iface
             import :: c_int, c_ptr, c_int32_t
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: case_pois
+        end function
+        integer(c_int) function mfa_hip_new_dpd_fd(ctx) bind(C, name="mfa_hip_new_dpd_fd")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
         end function
         integer(c_int) function mfa_hip_wall_wall(ctx, inter_type, sigma_w, v) bind(C, name="mfa_hip_wall_wall")
             import :: c_int, c_ptr, c_int32_t, c_double
@@ -207,7 +212,13 @@ contains
         p%poiseuille = 0
 #endif
         p%l_box = L_BOX
-        p%reserved0 = 0
+        p%bend_form = 0
+#ifdef DPD_VV
+        p%dpd_vv = 1
+#else
+        p%dpd_vv = 0
+#endif
+        p%reserved1 = 0
         p%boundary = boundary
         k = 0
         do j = 1, 4
@@ -340,3 +351,10 @@ subroutine verlet_velocities()                    ! verlet_velocities.f90:1 (+ e
     t_total = t_wall + t_fluid
     e_total = v_total + t_total
 end subroutine verlet_velocities
+
+subroutine new_dpd_fd()                           ! new_dpd_fd.f90:1 (md_main.f90:198, DPD_VV)
+    use mfa_hip_state
+    call mfa_bridge_ensure()
+    call mfa_chk(mfa_hip_new_dpd_fd(mfa_ctx), "new_dpd_fd")
+    call mfa_bridge_pull(mod(mfa_step_count, mfa_sync_every) == 0)    ! force and a changed again
+end subroutine new_dpd_fd

@@ -47,7 +47,11 @@ class Flags:
     FORCE_SWITCH_ON (:73), poiseuille POISEUILLE (:80), l_box L_BOX (:105),
     langevin THERMOSTAT==1 with the Langevin bead forces of lgv_forces.f90
     (used when thermostat is off; the reference's current fluid_fluid.f90
-    leaves its call commented out, :455-458).
+    leaves its call commented out, :455-458), bend_cosine: the bending terms
+    take the cosine form k_bend (1 - cos theta) instead of the reference's
+    harmonic angle (bending.f90:97-100), dpd_vv DPD_VV (:65): the
+    dissipative forces are evaluated again after verlet_velocities
+    (new_dpd_fd.f90).
     WALL is fixed to 2 (implicit 9-3 walls) and SYMMETRY to 0 (channel).
     """
     solvent: int = 3
@@ -61,6 +65,8 @@ class Flags:
     poiseuille: bool = True
     l_box: int = 1
     langevin: bool = False
+    bend_cosine: bool = False
+    dpd_vv: bool = False
 
 
 @dataclass

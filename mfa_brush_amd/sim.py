@@ -30,6 +30,7 @@ def to_c(p: SimParams, t_idx0: int = 0) -> _lib.SystemParams:
     s.force_switch_on = int(fl.force_switch_on)
     s.bending, s.bending_melt, s.poiseuille = int(fl.bending), int(fl.bending_melt), int(fl.poiseuille)
     s.l_box = fl.l_box
+    s.bend_form = 1 if getattr(fl, "bend_cosine", False) else 0
     s.boundary = _lib._D3(*p.boundary)
     for name in ("epsil", "sigma_2", "range_2", "e_shift", "inv_range_2"):
         setattr(s, name, _lib._D16(*[float(x) for x in getattr(p, name)]))
@@ -47,6 +48,7 @@ def to_c(p: SimParams, t_idx0: int = 0) -> _lib.SystemParams:
     s.mass_type = _lib._D4(*p.mass_type)
     s.seed = p.seed
     s.t_idx0 = t_idx0
+    s.dpd_vv = 1 if getattr(fl, "dpd_vv", False) else 0
     return s
 
 
@@ -145,6 +147,10 @@ class Simulation:
     def verlet_velocities(self):
         self._c(self.lib.mfa_hip_verlet_velocities(self.h), "verlet_velocities")
 
+    def new_dpd_fd(self):
+        """new_dpd_fd.f90 (DPD_VV): dissipative forces at the new velocities."""
+        self._c(self.lib.mfa_hip_new_dpd_fd(self.h), "new_dpd_fd")
+
     def step_by_routines(self):
         """One md_main.f90:77-185 iteration through the single-routine entry points."""
         p = self.params[0]
@@ -159,6 +165,8 @@ class Simulation:
         if p.flags.poiseuille:
             self.constant_force(1)
         self.verlet_velocities()
+        if getattr(p.flags, "dpd_vv", False) and p.flags.thermostat:
+            self.new_dpd_fd()
 
     def md_steps(self, n: int, graph: bool = False):
         self._c(self.lib.mfa_hip_md_steps(self.h, n, int(graph)), "md_steps")

@@ -662,7 +662,9 @@ double orc_intra_molec(const orc_params *p, const double *r0, double *force, int
 }
 
 /* bending(1), bending.f90:19-121 (melt=0) / bending_melt(1),
- * bending_melt.f90:22-120 (melt=1; the missing `beg+` of :43 is restored). */
+ * bending_melt.f90:22-120 (melt=1; the missing `beg+` of :43 is restored).
+ * bend_form 1 replaces the harmonic angle term by k_bend (1 - cos theta) over
+ * the same angles (no reference counterpart; the melt workload's form). */
 double orc_bending(const orc_params *p, const double *r0, double *force, int melt) {
     int first = melt ? p->n_mon * p->n_chain : 0;
     int n_mon = melt ? p->n_mon_d : p->n_mon;
@@ -713,6 +715,22 @@ double orc_bending(const orc_params *p, const double *r0, double *force, int mel
             } else {
                 double s = sqrt(dp2 * rp2);
                 for (int d = 0; d < 3; ++d) dp[d] = dp[d] / s;
+            }
+            if (p->bend_form == 1) {
+                /* cosine form V = k (1 - cos theta), theta the angle between
+                 * consecutive bonds (0 straight): F_prev = k dcos/dr_prev =
+                 * k (cos rp/|rp|^2 - rn/(|rp||rn|)), F_next = k (rp/(|rp||rn|)
+                 * - cos rn/|rn|^2), the centre takes minus their sum */
+                double inv_pn = 1.0 / sqrt(rp2 * rn2);
+                vb = vb + p->k_bend * (1.0 - cosa);
+                for (int d = 0; d < 3; ++d) {
+                    double fp = p->k_bend * (cosa * rp[d] / rp2 - rn[d] * inv_pn);
+                    double fn = p->k_bend * (rp[d] * inv_pn - cosa * rn[d] / rn2);
+                    force[3 * (ic - 1) + d] += fp;
+                    force[3 * ic + d] = force[3 * ic + d] - fp - fn;
+                    force[3 * (ic + 1) + d] += fn;
+                }
+                continue;
             }
             double dalpha = cosa >= 1.0 ? 0.0 : acos(cosa) - p->alpha_eq;
             double fmod = p->k_bend * dalpha;
@@ -854,6 +872,56 @@ void orc_cells_free(orc_cells *c) {
     memset(c, 0, sizeof(*c));
 }
 
+/* Dissipative part of one pair (i, j), as pair_body decides and weighs it:
+ * with DPD_CUT_OFF the non-head pairs within rc_dpd (their own 1/r), else the
+ * pairs within the LJ range (with the switch-on clamp of r^2). */
+static inline void dpd_d_pair(const orc_params *p, const double *r0, const double *v, const int *a_type,
+                              int i, int j, double r_2_min, double *force_d) {
+    int ti = a_type[i], tj = a_type[j];
+    double dr[3];
+    for (int d = 0; d < 3; ++d) dr[d] = r0[3 * i + d] - r0[3 * j + d];
+    dr[0] = mic(dr[0], p->boundary[0], p->inv_boundary[0]);
+    dr[1] = mic(dr[1], p->boundary[1], p->inv_boundary[1]);
+    double r_2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    int lj = r_2 < p->range_2[T2(ti, tj)];
+    double w;
+    if (p->dpd_cut_off) {
+        if (ti == 1 || tj == 1 || !(r_2 < p->r_cut_dpd_2)) return;
+        if (lj && p->force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+        w = 1.0 - sqrt(r_2 / p->r_cut_dpd_2);
+    } else {
+        if (!lj) return;
+        if (p->force_switch_on && r_2 < r_2_min) r_2 = r_2_min;
+        w = 1.0 - sqrt(r_2 * p->inv_range_2[T2(ti, tj)]);
+    }
+    double inv = sqrt(1.0 / r_2), rv[3];
+    for (int d = 0; d < 3; ++d) rv[d] = dr[d] * inv;
+    double rd = (v[3 * i] - v[3 * j]) * rv[0] + (v[3 * i + 1] - v[3 * j + 1]) * rv[1] +
+                (v[3 * i + 2] - v[3 * j + 2]) * rv[2];
+    for (int d = 0; d < 3; ++d) {
+        double f = -1.0 * p->friction * (w * w) * rd * rv[d];
+        force_d[3 * i + d] += f;
+        force_d[3 * j + d] -= f;
+    }
+}
+
+/* new_dpd_fd.f90:19-115: the dissipative forces over every pair of the cell
+ * list's half shell (each unordered pair once). */
+void orc_dpd_dissipative(const orc_params *p, const orc_cells *c, const double *r0, const double *v,
+                         const int *a_type, double r_2_min, double *force_d) {
+    for (int ic = 1; ic <= c->n_cells_tot; ++ic) {
+        for (int ip = c->part_in_cell[ic]; ip != 0; ip = c->r_nei[ip - 1]) {
+            for (int jd = 0; jd < c->n_nei; ++jd) {
+                int jc = c->cell_neigh_ls[(ic - 1) + (size_t)c->n_cells_tot * jd];
+                for (int jp = c->part_in_cell[jc]; jp != 0; jp = c->r_nei[jp - 1])
+                    dpd_d_pair(p, r0, v, a_type, ip - 1, jp - 1, r_2_min, force_d);
+            }
+            for (int jp = c->part_in_cell[ic]; jp != 0; jp = c->r_nei[jp - 1])
+                if (ip < jp) dpd_d_pair(p, r0, v, a_type, ip - 1, jp - 1, r_2_min, force_d);
+        }
+    }
+}
+
 void orc_md_steps(const orc_params *p, orc_cells *c, double *r0, double *v, double *a,
                   double *force, const int *a_type, const double *mass, const double *inv_mass,
                   const double *ext_force, long long t_idx0, int n_steps, int noise,
@@ -873,7 +941,35 @@ void orc_md_steps(const orc_params *p, orc_cells *c, double *r0, double *v, doub
         e->v_bend_melt = p->bending_melt ? orc_bending(p, r0, force, 1) : 0.0;
         if (ext_force)
             for (size_t k = 0; k < n3; ++k) force[k] += ext_force[k];
+        double *v_half = NULL;
+        if (p->dpd_vv && p->thermostat == 1) {
+            v_half = (double *)malloc(n3 * sizeof(double));
+            memcpy(v_half, v, n3 * sizeof(double));
+        }
         e->t_fluid = orc_verlet_velocities(p, force, inv_mass, mass, v, a, f_on_heads);
+        if (v_half) {
+            /* new_dpd_fd.f90 (md_main.f90:187-198): the dissipative part of
+             * the force is evaluated again at the new velocities and replaces
+             * the one of the half-step velocities; the accelerations follow.
+             * (The reference's routine also subtracts a force_d its pair
+             * routine never fills and applies a further half kick; the
+             * published DPD-VV scheme, Besold et al. PRE 62 R7611 (2000),
+             * only replaces F_D, as here.) */
+            double *fd = (double *)calloc(2 * n3, sizeof(double));
+            orc_dpd_dissipative(p, c, r0, v_half, a_type, r_2_min, fd);
+            orc_dpd_dissipative(p, c, r0, v, a_type, r_2_min, fd + n3);
+            for (int i = 0; i < p->n_part; ++i) {
+                const int head = i < p->n_mon * p->n_chain && i % p->n_mon == 0;
+                for (int d = 0; d < 3; ++d) {
+                    size_t k = 3 * (size_t)i + d;
+                    if (head) continue;            /* heads keep force and a zero */
+                    force[k] = force[k] + (fd[n3 + k] - fd[k]);
+                    a[k] = force[k] * inv_mass[i];
+                }
+            }
+            free(fd);
+            free(v_half);
+        }
     }
 }
 

@@ -57,6 +57,8 @@ typedef struct {
     double head_vel[2];
     double mass_type[4];
     uint64_t seed;         /* Philox key for the DPD noise */
+    int bend_form;         /* 0: harmonic angle, bending.f90:97-100; 1: cosine k_bend (1 - cos theta) */
+    int dpd_vv;            /* DPD_VV: dissipative forces re-evaluated after verlet_velocities (new_dpd_fd.f90) */
 } orc_params;
 
 /* ---- RNG: Philox-4x32-10 (Salmon et al., SC'11) and the pair normal ---- */
@@ -132,6 +134,10 @@ void orc_md_steps(const orc_params *p, orc_cells *c, double *r0, double *v, doub
                   double *force, const int *a_type, const double *mass, const double *inv_mass,
                   const double *ext_force, long long t_idx0, int n_steps, int noise,
                   orc_energies *e, double *f_on_heads);
+/* dissipative DPD forces of every DPD pair at velocities v (the pair set and
+ * weights of fluid_fluid's DPD part at this r_2_min), added into force_d */
+void orc_dpd_dissipative(const orc_params *p, const orc_cells *c, const double *r0, const double *v,
+                         const int *a_type, double r_2_min, double *force_d);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

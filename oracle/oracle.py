@@ -38,7 +38,7 @@ class OrcParams(C.Structure):
         ("k_bend", C.c_double), ("alpha_eq", C.c_double),
         ("dt", C.c_double), ("dt_2", C.c_double),
         ("const_force", _D3), ("head_vel", _D2), ("mass_type", _D4),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("bend_form", C.c_int), ("dpd_vv", C.c_int),
     ]
 
 
@@ -133,6 +133,8 @@ def make_params(p) -> OrcParams:
     o.head_vel = _D2(*p.head_vel)
     o.mass_type = _D4(*p.mass_type)
     o.seed = p.seed
+    o.bend_form = 1 if getattr(fl, "bend_cosine", False) else 0
+    o.dpd_vv = 1 if getattr(fl, "dpd_vv", False) else 0
     return o
 
 

@@ -507,11 +507,13 @@ def test_error_paths():
             s.fluid_wall(1)
 
 
+@pytest.mark.parametrize("cosine", [True, False])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_melt_channel_vs_oracle(oracle_mod, precision):
-    """Melt without a brush (n_chain = 0): FENE + bending_melt + Poiseuille
-    body force + DPD, 10 fused steps against the oracle."""
-    p, t, r0, v = configs.channel_melt(n_cols=8, n_z=32, skin=0.3)
+def test_melt_channel_vs_oracle(oracle_mod, precision, cosine):
+    """Melt without a brush (n_chain = 0): FENE + bending_melt (cosine and
+    harmonic forms) + Poiseuille body force + DPD, 10 fused steps against
+    the oracle."""
+    p, t, r0, v = configs.channel_melt(n_cols=8, n_z=32, skin=0.3, bend_cosine=cosine)
     o = _oracle(oracle_mod, p)
     a0 = np.zeros_like(r0)
     ro, vo, ao, fo, eo, _ = o.md_steps(r0, v, a0, t, 0, 10, noise=1)

@@ -131,3 +131,101 @@ def test_pair_timing_same_bits():
             assert n == 0
     for k in ("r0", "v", "a", "force"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cosine_bending_routines_and_fused(oracle_mod, precision):
+    """Cosine bending k_b (1 - cos theta) of brush (bending) and melt
+    (bending_melt) chains: the routine-level entry points against the
+    oracle's forces and energies, and 10 fused steps against the oracle."""
+    from mfa_brush_amd.params import Flags
+    p = configs.channel_params((16.0, 16.0, 20.0), 16, 128, 256, thermostat=True, rc_dpd=1.0, bending=True,
+                               k_bend=5.0, const_force=(0.01, 0.0, 0.0))
+    p = p.replace(n_mon_d=8, r_2_min_init=0.0, r_2_min_time=0.0,
+                  flags=Flags(**{**p.flags.__dict__, "bending_melt": True, "bend_cosine": True}))
+    t, r0, v = configs.build_channel(p, 2.0, 8)
+    rng = np.random.default_rng(4)
+    r0 = r0 + rng.normal(0, 0.05, r0.shape)
+    r0[:, :2] %= 16.0
+    o = oracle_mod.Oracle(p)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        f0 = s.get_state()["force"]
+        s.bending(1)
+        f1 = s.get_state()["force"]
+        s.bending_melt(1)
+        f2 = s.get_state()["force"]
+        e = s.energies()
+    fb, eb = o.bending(r0, 0)
+    fm, em = o.bending(r0, 1)
+    for got, want in ((f1 - f0, fb), (f2 - f1, fm)):
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    assert e["v_bend"] == pytest.approx(eb, rel=1e-11) and e["v_bend_melt"] == pytest.approx(em, rel=1e-11)
+    ro, vo, ao, fo, eo, _ = o.md_steps(r0, v, np.zeros_like(r0), t, 0, 10, noise=1)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t)
+        s.md_steps(10, graph=True)
+        st = s.get_state()
+        e = s.energies()
+    tol = 1e-9 if precision == "fp64" else 1e-4
+    assert np.abs(st["r0"] - ro).max() <= tol * np.abs(ro).max()
+    for k in ("v_bend", "v_bend_melt"):
+        assert e[k] == pytest.approx(eo[k], rel=1e3 * tol), k
+
+
+def _dpd_vv_c1(rc_dpd, skin=0.3):
+    from mfa_brush_amd.params import Flags
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=rc_dpd)
+    heads = np.arange(p.n_chain) * p.n_mon
+    v = v.copy()
+    v[heads] = 0.0
+    p = p.replace(skin=skin, flags=Flags(**{**p.flags.__dict__, "dpd_vv": True}))
+    return p, t, r0, v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("rc_dpd", [1.0, None])
+def test_dpd_vv_vs_oracle(oracle_mod, precision, rc_dpd):
+    """DPD-VV (new_dpd_fd.f90): 10 fused steps, graph-replayed, against the
+    oracle's steps, which evaluate the dissipative forces again at the new
+    velocities; the device adds F_D of the kick dt/2 a instead (the force is
+    linear in the velocities).  The routine-level new_dpd_fd after
+    verlet_velocities gives the fused step's state."""
+    p, t, r0, v = _dpd_vv_c1(rc_dpd)
+    o = oracle_mod.Oracle(p)
+    a0 = np.zeros_like(r0)
+    ro, vo, ao, fo, eo, _ = o.md_steps(r0, v, a0, t, 0, 10, noise=1)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t, a0)
+        s.md_steps(10, graph=True)
+        st = s.get_state()
+    tol = 1e-9 if precision == "fp64" else 1e-4
+    for k, want in (("r0", ro), ("v", vo), ("a", ao), ("force", fo)):
+        err = np.abs(st[k] - want).max() / np.abs(want).max()
+        assert err <= (tol if k == "r0" else 1e3 * tol), (k, err)
+    with Simulation(p, precision=precision) as s1, Simulation(p, precision=precision) as s2:
+        s1.set_state(r0, v, t, a0)
+        s2.set_state(r0, v, t, a0)
+        for _ in range(3):
+            s1.step_by_routines()
+        s2.md_steps(3)
+        a, b = s1.get_state(), s2.get_state()
+    for k in ("r0", "v", "a", "force"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * np.abs(b[k]).max(), k
+
+
+def test_dpd_vv_thermostat_holds_kt():
+    """DPD-VV keeps the DPD thermostat's temperature: kT = 1 within 5 % on C1
+    with DPD on every LJ pair."""
+    p, t, r0, v = _dpd_vv_c1(None, skin=0.4)
+    with Simulation(p, precision="mixed") as s:
+        s.set_state(r0, v, t)
+        s.md_steps(400, graph=True)
+        kes = []
+        for _ in range(20):
+            s.md_steps(20, graph=True)
+            kes.append(s.energies()["t_fluid"])
+        assert s.flags()["nonfinite"] == 0
+    kt = 2.0 * np.mean(kes) / (3 * (p.n_part - p.n_chain))
+    assert abs(kt - 1.0) < 0.05, kt

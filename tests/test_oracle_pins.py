@@ -297,3 +297,92 @@ def test_virial_cluster_identity_with_dpd(oracle_mod, noise):
     f_lj, _ = o.fluid_fluid_brute(r0, np.zeros_like(v), t, noise=0)
     assert np.abs(f - f_lj).max() > 0.1                # DPD forces present
     assert np.allclose(w, want, rtol=0, atol=1e-10 * np.abs(want).max())
+
+
+# ------------------------------------------------ cosine bending form ----
+def _cosine(p):
+    return p.replace(flags=Flags(**{**p.flags.__dict__, "bend_cosine": True}))
+
+
+@pytest.mark.parametrize("melt", [0, 1])
+def test_cosine_bending_is_minus_gradient(oracle_mod, melt):
+    p, _, r0, t = _melt_system(oracle_mod)
+    o = oracle_mod.Oracle(_cosine(p))
+    f, e = o.bending(r0, melt)
+    lo, hi = (20, len(r0)) if melt else (0, 20)
+    assert np.abs(f[lo:hi]).max() > 0.1
+    mask = np.ones(len(r0), bool); mask[lo:hi] = False
+    assert not f[mask].any()
+    h = 1e-6
+    for i in range(lo, hi):
+        for d in range(3):
+            rp = r0.copy(); rp[i, d] += h
+            rm = r0.copy(); rm[i, d] -= h
+            g = (o.bending(rp, melt)[1] - o.bending(rm, melt)[1]) / (2 * h)
+            assert -g == pytest.approx(f[i, d], rel=2e-5, abs=2e-6)
+    assert np.abs(f.sum(0)).max() < 1e-12 * max(1.0, np.abs(f).max())
+
+
+def test_cosine_bending_closed_forms(oracle_mod):
+    """k (1 - cos theta): straight chains 0, a right angle k with end forces
+    k / |b| perpendicular to their bonds, a full fold 2k with zero force
+    (the extremum), and the harmonic form's small-angle limit: for theta
+    -> 0, k (1 - cos theta) -> k theta^2 / 2, so both forms' forces agree to
+    O(theta^2)."""
+    p = configs.channel_params((12.0, 12.0, 14.0), 3, 3, 1, thermostat=False, bending=True, k_bend=5.0)
+    oc = oracle_mod.Oracle(_cosine(p))
+    th = 0.02
+    r0 = np.array([[1, 1, 2.0], [1, 1, 3.0], [1, 1, 4.0],                       # straight
+                   [5, 5, 10.0], [5, 5, 9.0], [6, 5, 9.0],                       # right angle
+                   [8, 8, 3.0], [8, 8, 4.0], [8, 8, 3.0 + 1e-9],                 # folded back
+                   [3, 9, 5.0]])
+    f, e = oc.bending(r0, 0)
+    want_e = 5.0 * 1.0 + 5.0 * 2.0
+    assert e == pytest.approx(want_e, rel=1e-9)
+    assert np.abs(f[0:3]).max() < 1e-12 and np.abs(f[6:9]).max() < 1e-6
+    for end, nb in ((3, 4), (5, 4)):
+        b = r0[end] - r0[nb]
+        assert abs(f[end].dot(b)) < 1e-12
+        assert np.linalg.norm(f[end]) == pytest.approx(5.0 / np.linalg.norm(b), rel=1e-12)
+    # small angle: cosine vs harmonic (alpha_eq = 0)
+    q = configs.channel_params((12.0, 12.0, 14.0), 3, 1, 1, thermostat=False, bending=True, k_bend=5.0)
+    r1 = np.array([[5, 5, 3.0], [5, 5, 4.0], [5 + math.sin(th), 5, 4.0 + math.cos(th)], [2, 2, 7.0]])
+    fc, ec = oracle_mod.Oracle(_cosine(q)).bending(r1, 0)
+    fh, eh = oracle_mod.Oracle(q).bending(r1, 0)
+    assert ec == pytest.approx(5.0 * (1 - math.cos(th)), rel=1e-9)
+    assert ec == pytest.approx(eh, rel=th * th)
+    assert np.abs(fc - fh).max() < th * th * np.abs(fh).max()
+
+
+# ------------------------------------------------------------- DPD-VV ----
+@pytest.mark.parametrize("rc_dpd", [1.0, None])
+def test_dpd_vv_force_is_pair_force_at_new_velocities(oracle_mod, rc_dpd):
+    """With DPD_VV the force an md step leaves behind is the total force with
+    the DPD pair part evaluated at the step's final velocities (same
+    positions, same noise): fluid_fluid at (r1, v1) plus walls, bonds and
+    body force, heads zero, and a = F / m (new_dpd_fd.f90).  Without DPD_VV
+    it is the force at the half-step velocities, which differs."""
+    p, t, r0, v = configs.channel_c1(thermostat=True, rc_dpd=rc_dpd)
+    rng = np.random.default_rng(3)
+    r0 = r0 + rng.normal(0, 0.05, r0.shape)
+    r0[:, :2] %= 16.0
+    heads = np.arange(p.n_chain) * p.n_mon
+    v = v.copy()
+    v[heads] = 0.0                                   # heads start at their fixed velocity
+    p = p.replace(flags=Flags(**{**p.flags.__dict__, "dpd_vv": True}), r_2_min_init=0.0, r_2_min_time=0.0)
+    o = oracle_mod.Oracle(p)
+    a0 = np.zeros_like(r0)
+    step = 7
+    r1, v1, a1, f1, _, _ = o.md_steps(r0, v, a0, t, step, 1, noise=1)
+    fp, _ = o.fluid_fluid_brute(r1, v1, t, step=step, noise=1)
+    fw, _ = o.fluid_wall(r1, t)
+    fb, _, _ = o.intra_molec(r1)
+    want = fp + fw + fb
+    want[heads] = 0.0
+    assert np.abs(f1 - want).max() <= 1e-11 * np.abs(want).max()
+    mass, inv = p.masses(t)
+    assert np.abs(a1 - f1 * inv[:, None]).max() <= 1e-14 * np.abs(a1).max()
+    q = p.replace(flags=Flags(**{**p.flags.__dict__, "dpd_vv": False}))
+    _, v2, _, f2, _, _ = oracle_mod.Oracle(q).md_steps(r0, v, a0, t, step, 1, noise=1)
+    assert np.array_equal(v2, v1)                    # DPD-VV leaves the velocities alone
+    assert np.abs(f2 - f1).max() > 1e-3 * np.abs(want).max()
