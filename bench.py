@@ -11,6 +11,11 @@ value = beads x steps / seconds of the mixed-precision path (FP32 pair
 arithmetic, FP64 integration and sums); the plain FP64 path is timed the same
 way and reported beside it.  The pair kernel's average launch duration comes
 from HIP events on the library's stream over the timed steps.
+
+time_to_equilibrium: a bounded run of the learned warm start (training
+profiles from 24 small batched channels, surrogate training, then a held-out
+12.8k-bead channel from the plain and from the warm start), blocks to the
+10 / 5 / 2 % profile bands for both starts.
 """
 import argparse
 import json
@@ -71,32 +76,6 @@ def run_gpu(precision, steps, warmup, ps, states, nlist=-1):
                 list_cap=cap, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
 
 
-def _c4_replica(i):
-    """Replica i of the batched workload: size log-uniform in [2k, 40k]
-    beads, grafting density U[0.05, 0.5], N in {8, 16, 32, 64}, brush-solvent
-    epsilon U[0.5, 1.5], DPD rc 1; per-replica seeds.  Lz is 30, raised
-    where straight chains from both walls would not fit (N 32 and 64) or the
-    two brushes alone would exceed 5/8 of the beads at density 0.8."""
-    import math
-    import numpy as np
-    from mfa_brush_amd import configs
-    rng = np.random.default_rng(1000 + i)
-    n_target = math.exp(rng.uniform(math.log(2000), math.log(40000)))
-    sg = rng.uniform(0.05, 0.5)
-    n_mon = int(rng.choice([8, 16, 32, 64]))
-    eps = rng.uniform(0.5, 1.5)
-    lz = max(30.0, 2.0 * sg * n_mon / 0.5, 2.0 * (1.2 + 0.97 * (n_mon - 1)) + 2.0)
-    a = 1.0 / math.sqrt(sg)
-    side = max(9.0, math.sqrt(n_target / (0.8 * lz)))
-    n_side = max(2, int(round(side / a)))
-    while n_side * a < 9.0:
-        n_side += 1
-    p, t, r0, v = configs.channel_general(n_side, a, n_mon, lz, eps_bs=eps, thermostat=True,
-                                          rc_dpd=1.0, seed=0x5EED0001 + i, force_switch_on=True)
-    # force switch-on ramp over the first 1 tau for crowded starts
-    return p.replace(r_2_min_init=0.64, r_2_min_time=1.0), t, r0, v
-
-
 def build_workload(name, skin, n_rep=256):
     """(params list, states list, description) of a bench workload."""
     from mfa_brush_amd import configs
@@ -109,9 +88,7 @@ def build_workload(name, skin, n_rep=256):
         return [p], [(t, r0, v)], ("melt channel 1,200,000 beads, 18,750 chains N=64, box 216.93x216.93x30, "
                                    "FENE + cosine bending k_b 5, Poiseuille g 0.01 x, DPD rc 1, dt 0.005")
     if name == "c4":
-        from concurrent.futures import ProcessPoolExecutor
-        with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-            reps = list(ex.map(_c4_replica, range(n_rep)))
+        reps = configs.c4_batch(n_rep)
         ps = [r[0].replace(skin=skin) for r in reps]
         return ps, [(r[1], r[2], r[3]) for r in reps], (
             "%d brush-channel replicas, 2k-40k beads log-uniform, grafting density U[0.05,0.5], N in "
@@ -174,6 +151,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="steps of the CPU baseline sample (10-20 s of CPU work at 1M beads on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tte", action="store_true",
+                    help="skip the bounded time-to-equilibrium run of the learned warm start")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-list", action="store_true", help="pair kernels scan the cells every step")
     ap.add_argument("--skin", type=float, default=0.4,
@@ -248,6 +227,10 @@ def main():
                        "pair_frac_fp64_peak": FLOPS_PER_PAIR * fp64["npairs"] / p64 / PEAK_FP64,
                        "kernels_ms": {k: d["total_ms"] / max(d["launches"], 1) for k, d in fp64["kernels"].items()}}
         out["mixed_over_fp64"] = value / v64
+    if not args.no_tte:
+        from mfa_brush_amd.warmstart import bounded_tte
+        out["time_to_equilibrium"] = bounded_tte()
+        print("time to equilibrium: %.1f s" % out["time_to_equilibrium"]["seconds"], file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ps, states, args.cpu_steps)
     if tuned is not None:

@@ -253,7 +253,13 @@ def _insert_solvent(L, lz, occupied, n_sol, rng, weight=None, dmin=0.85, z_margi
     """Random sequential insertion of n_sol beads in [0,L)^2 x [z_margin,
     lz - z_margin], each at least dmin (minimum image in x, y) from the
     occupied beads and from the beads inserted before it.  Heights are
-    uniform, or drawn from weight(z) (inverse CDF on a 0.05 sigma grid)."""
+    uniform, or drawn from weight(z) (inverse CDF on a 0.05 sigma grid).
+
+    Candidates come in batches of 4096 and are taken in order: a candidate
+    is kept when it is clear of the occupied beads, of the beads kept from
+    earlier batches (k-d trees) and of the candidates kept before it in its
+    own batch (only the few close pairs inside a batch are walked in order),
+    which is the sequential insertion rule."""
     from scipy.spatial import cKDTree
     zs = np.arange(z_margin, lz - z_margin, 0.05)
     cdf = None
@@ -261,11 +267,15 @@ def _insert_solvent(L, lz, occupied, n_sol, rng, weight=None, dmin=0.85, z_margi
         w = np.maximum(np.asarray([weight(z) for z in zs], float), 1e-9)
         cdf = np.cumsum(w)
         cdf /= cdf[-1]
-    tree = cKDTree(np.c_[occupied[:, 0] % L, occupied[:, 1] % L, occupied[:, 2] + lz], boxsize=[L, L, 4 * lz]) \
-        if len(occupied) else None
-    hsh = _Hash(L)
+    box = [L, L, 4 * lz]                 # periodic in x, y; z shifted by lz never wraps
+
+    def lift(q):
+        return np.c_[q[:, 0] % L, q[:, 1] % L, q[:, 2] + lz]
+
+    tree = cKDTree(lift(occupied), boxsize=box) if len(occupied) else None
     out = np.zeros((n_sol, 3))
     k, tries = 0, 0
+    kept_tree, kept_n = None, 0
     while k < n_sol:
         if tries > max_tries * max(n_sol, 1) // 4096 + 50:
             raise ValueError("solvent insertion stalled at %d of %d beads" % (k, n_sol))
@@ -274,15 +284,26 @@ def _insert_solvent(L, lz, occupied, n_sol, rng, weight=None, dmin=0.85, z_margi
         cand = np.c_[rng.uniform(0, L, m), rng.uniform(0, L, m),
                      rng.uniform(z_margin, lz - z_margin, m) if cdf is None else np.interp(rng.uniform(0, 1, m), cdf, zs)]
         if tree is not None:
-            d, _ = tree.query(np.c_[cand[:, 0], cand[:, 1], cand[:, 2] + lz], k=1)
+            d, _ = tree.query(lift(cand), k=1)
             cand = cand[d >= dmin]
-        for r in cand:
-            if hsh.clear_of(r, dmin):
-                hsh.add(r)
-                out[k] = r
-                k += 1
-                if k == n_sol:
-                    break
+        if kept_n != k:
+            kept_tree, kept_n = cKDTree(lift(out[:k]), boxsize=box), k
+        if kept_tree is not None and len(cand):
+            d, _ = kept_tree.query(lift(cand), k=1)
+            cand = cand[d >= dmin]
+        if not len(cand):
+            continue
+        # conflicts inside the batch: candidate j loses to an earlier kept i
+        pairs = cKDTree(lift(cand), boxsize=box).query_pairs(dmin, output_type="ndarray")
+        earlier = [[] for _ in range(len(cand))]
+        for i, j in pairs:
+            earlier[max(i, j)].append(min(i, j))
+        keep = np.zeros(len(cand), bool)
+        for j in range(len(cand)):
+            keep[j] = not any(keep[i] for i in earlier[j])
+        take = cand[keep][:n_sol - k]
+        out[k:k + len(take)] = take
+        k += len(take)
     return out
 
 
@@ -401,3 +422,34 @@ def channel_melt(n_cols: int = 200, n_z: int = 30, n_mon: int = 64, lz: float = 
     a_type = np.full(n_sites, 3, np.int32)
     v = rng.normal(0.0, 1.0, size=(n_sites, 3))
     return p, a_type, r0, v
+
+
+def c4_replica(i: int):
+    """Replica i of the batched workload: size log-uniform in [2k, 40k]
+    beads, grafting density U[0.05, 0.5], N in {8, 16, 32, 64}, brush-solvent
+    epsilon U[0.5, 1.5], DPD rc 1; per-replica seeds.  Lz is 30, raised
+    where straight chains from both walls would not fit (N 32 and 64) or the
+    two brushes alone would exceed 5/8 of the beads at density 0.8.  The
+    crowded starts take the force switch-on ramp over the first 1 tau."""
+    rng = np.random.default_rng(1000 + i)
+    n_target = math.exp(rng.uniform(math.log(2000), math.log(40000)))
+    sg = rng.uniform(0.05, 0.5)
+    n_mon = int(rng.choice([8, 16, 32, 64]))
+    eps = rng.uniform(0.5, 1.5)
+    lz = max(30.0, 2.0 * sg * n_mon / 0.5, 2.0 * (1.2 + 0.97 * (n_mon - 1)) + 2.0)
+    a = 1.0 / math.sqrt(sg)
+    side = max(9.0, math.sqrt(n_target / (0.8 * lz)))
+    n_side = max(2, int(round(side / a)))
+    while n_side * a < 9.0:
+        n_side += 1
+    p, t, r0, v = channel_general(n_side, a, n_mon, lz, eps_bs=eps, thermostat=True,
+                                  rc_dpd=1.0, seed=0x5EED0001 + i, force_switch_on=True)
+    return p.replace(r_2_min_init=0.64, r_2_min_time=1.0), t, r0, v
+
+
+def c4_batch(n_rep: int = 256, workers: int = 16):
+    """The first n_rep replicas of the batched workload, built in parallel."""
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=max(1, min(workers, os.cpu_count() or 1))) as ex:
+        return list(ex.map(c4_replica, range(n_rep)))

@@ -200,6 +200,44 @@ def time_to_equilibrium(blocks, ref, tol: float = 0.02, consecutive: int = 5):
     return None, err
 
 
+def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 40, block: int = 500,
+                mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21):
+    """The whole warm-start method at a bounded size (tens of seconds on one
+    GPU): n_train small channels (12 sigma boxes, N in {8, 16}) run batched
+    for equil + avg steps to give training profiles, the surrogate trains on
+    them, and a held-out channel runs n_blocks blocks of `block` steps from
+    the plain and from the warm start.  Returns the first-block L1 errors,
+    the surrogate's own L1 error and the blocks to the 10 / 5 / 2 % bands
+    (5 consecutive blocks) of both starts, against the mean of the second
+    halves of both runs."""
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    specs = [Spec(sigma_g=float(rng.uniform(0.05, 0.5)), n_mon=int(rng.choice([8, 16])),
+                  eps_bs=float(rng.uniform(0.5, 1.5)), side=12.0, seed=300 + i) for i in range(n_train)]
+    profs, _, n_beads = equilibrium_profiles(specs, equil, avg)
+    t_sim = time.perf_counter() - t0
+    mlp, hist = train(specs, profs, steps=mlp_steps)
+    held = held or Spec(0.25, 16, 1.0, side=20.0, seed=907)
+    plain, n_part, secs_p, flp = block_profiles(held, None, n_blocks, block=block)
+    warm, _, secs_w, flw = block_profiles(held, mlp, n_blocks, block=block)
+    late_p, late_w = plain[n_blocks // 2:].mean(0), warm[n_blocks // 2:].mean(0)
+    ref = 0.5 * (late_p + late_w)
+    zc, rb, rs = predict(mlp, held)
+    out = {"held_out": asdict(held), "n_part": n_part, "block_steps": block, "blocks": n_blocks,
+           "train_channels": n_train, "train_beads": n_beads, "train_steps": equil + avg,
+           "mlp_loss_first_last": [float(hist[:20].mean()), float(hist[-20:].mean())],
+           "first_block_l1_plain": l1_error(plain[0], ref), "first_block_l1_warm": l1_error(warm[0], ref),
+           "surrogate_l1": l1_error(np.stack([rb, rs]), ref), "late_gap_l1": l1_error(late_w, late_p),
+           "nonfinite": int(flp["nonfinite"] + flw["nonfinite"]), "seconds": time.perf_counter() - t0,
+           "train_sim_seconds": t_sim}
+    for tol in (0.10, 0.05, 0.02):
+        tp, _ = time_to_equilibrium(plain, ref, tol=tol)
+        tw, _ = time_to_equilibrium(warm, ref, tol=tol)
+        out["tte_blocks_%d%%" % round(100 * tol)] = {"plain": tp, "warm": tw,
+                                                      "speedup": (tp / tw) if (tp and tw) else None}
+    return out
+
+
 def main(argv=None):
     """Train on batched runs, then compare plain and warm starts on held-out
     channels.  Arguments: report path, training channels, equilibration and
