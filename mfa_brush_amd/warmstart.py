@@ -186,6 +186,29 @@ def block_profiles(spec: Spec, mlp, n_blocks: int, block: int = 1000, every: int
     return np.array(out), p.n_part, time.perf_counter() - t0, fl
 
 
+def block_profiles_both(spec: Spec, mlp, n_blocks: int, block: int = 1000, every: int = 50, skin: float = 0.3):
+    """Block-averaged (brush, solvent) densities of the plain start and the
+    warm start of one channel, run side by side in one batched context (each
+    channel's trajectory is the one it has alone)."""
+    starts = [build(spec, None, skin), build(spec, mlp, skin)]
+    out = [[], []]
+    t0 = time.perf_counter()
+    with Simulation([b[0] for b in starts]) as sim:
+        for i, (p, t, r0, v) in enumerate(starts):
+            sim.set_state(r0, v, t, isys=i)
+        for b in range(n_blocks):
+            sim.profiles_reset(N_BINS)
+            for q in range(block // every):
+                sim.md_steps(every, graph=True)
+                sim.profiles_accumulate()
+            for i, (p, *_r) in enumerate(starts):
+                n_obs, dens, _, _ = sim.profiles(i)
+                out[i].append(densities(p, dens, n_obs))
+        sim.synchronize()
+        fl = sim.flags()
+    return np.array(out[0]), np.array(out[1]), starts[0][0].n_part, time.perf_counter() - t0, fl
+
+
 def l1_error(prof, ref):
     return float(np.abs(prof - ref).sum() / np.abs(ref).sum())
 
@@ -200,7 +223,7 @@ def time_to_equilibrium(blocks, ref, tol: float = 0.02, consecutive: int = 5):
     return None, err
 
 
-def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 40, block: int = 500,
+def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 100, block: int = 1000,
                 mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21):
     """The whole warm-start method at a bounded size (tens of seconds on one
     GPU): n_train small channels (12 sigma boxes, N in {8, 16}) run batched
@@ -218,18 +241,21 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
     t_sim = time.perf_counter() - t0
     mlp, hist = train(specs, profs, steps=mlp_steps)
     held = held or Spec(0.25, 16, 1.0, side=20.0, seed=907)
-    plain, n_part, secs_p, flp = block_profiles(held, None, n_blocks, block=block)
-    warm, _, secs_w, flw = block_profiles(held, mlp, n_blocks, block=block)
+    plain, warm, n_part, secs, fl = block_profiles_both(held, mlp, n_blocks, block=block)
     late_p, late_w = plain[n_blocks // 2:].mean(0), warm[n_blocks // 2:].mean(0)
     ref = 0.5 * (late_p + late_w)
     zc, rb, rs = predict(mlp, held)
+    err_p = [l1_error(b, ref) for b in plain]
+    err_w = [l1_error(b, ref) for b in warm]
     out = {"held_out": asdict(held), "n_part": n_part, "block_steps": block, "blocks": n_blocks,
+           "l1_every_10th_block_plain": [round(e, 4) for e in err_p[::10]],
+           "l1_every_10th_block_warm": [round(e, 4) for e in err_w[::10]],
            "train_channels": n_train, "train_beads": n_beads, "train_steps": equil + avg,
            "mlp_loss_first_last": [float(hist[:20].mean()), float(hist[-20:].mean())],
            "first_block_l1_plain": l1_error(plain[0], ref), "first_block_l1_warm": l1_error(warm[0], ref),
            "surrogate_l1": l1_error(np.stack([rb, rs]), ref), "late_gap_l1": l1_error(late_w, late_p),
-           "nonfinite": int(flp["nonfinite"] + flw["nonfinite"]), "seconds": time.perf_counter() - t0,
-           "train_sim_seconds": t_sim}
+           "nonfinite": int(fl["nonfinite"]), "seconds": time.perf_counter() - t0,
+           "train_sim_seconds": t_sim, "held_out_seconds": secs}
     for tol in (0.10, 0.05, 0.02):
         tp, _ = time_to_equilibrium(plain, ref, tol=tol)
         tw, _ = time_to_equilibrium(warm, ref, tol=tol)

@@ -65,7 +65,7 @@ def test_warm_start_beats_plain_start():
     block of MD is at least 2x closer (L1) to the equilibrium profiles than
     the plain straight-chain start's first block."""
     from mfa_brush_amd.warmstart import bounded_tte
-    r = bounded_tte()
+    r = bounded_tte(n_blocks=30)
     print(r)
     assert r["nonfinite"] == 0
     assert r["surrogate_l1"] < 0.3, r["surrogate_l1"]
