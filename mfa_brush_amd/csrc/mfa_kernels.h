@@ -123,12 +123,16 @@ __device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Philox-4x32-10 (Salmon et al., SC'11)
+// Philox-4x32-10 (Salmon et al., SC'11).  Each 32x32 product is taken as one
+// 64-bit product (one v_mad_u64_u32) rather than separate high and low
+// halves (two quarter-rate multiplies).
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         uint32_t n0 = hi1 ^ c[1] ^ k0;
         uint32_t n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;

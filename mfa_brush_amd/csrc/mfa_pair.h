@@ -551,22 +551,29 @@ __device__ __forceinline__ Sep32 sep32(float4 pf, float4 pj, float ox, float oy,
     return d;
 }
 
+// Per-channel pair tables of the mixed kernel in LDS, one row of 8 entries
+// per bead type and one entry per partner type bits (the low 3 bits of a
+// posf identity word, types 1..4), so a partner's entry is
+// T[field * 32 + (((ti - 1) & 3) * 8 | (w & 7))]: one index per candidate
+// for every field.
+enum T32Field { F_REACH = 0, F_LO, F_HI, F_EPS, F_SIG2, F_ESH, F_DPD, F_IRC, F_NF };
+
 // LJ part of candidate pj (slot m) for bead k, and whether the pair takes a
 // DPD term: 0 none, 1 within the LJ range, 2 beyond it (DPD_CUT_OFF only).
+// ix: the candidate's table index.  The early exit compares r^2 with the
+// reach, max(LJ range widened by 2^-14, DPD range widened by 1e-4).
 template <bool DPD, bool VIR>
-__device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const Pair32Ctx &X,
+__device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const Pair32Ctx &X,
                                          const double4 *__restrict__ pos, int k, int ti, float4 pf,
-                                         float ox, float oy, float oz, float4 pj, int m, PairAcc &a) {
+                                         float ox, float oy, float oz, float4 pj, int m, int ix, PairAcc &a) {
 #pragma clang fp contract(off)
     const Sep32 d = sep32(pf, pj, ox, oy, oz);
     float r2 = d.r2;
-    const int tj = fwtype(pj.w);
-    const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
-    const bool maybe_dpd = DPD && X.dpd_cut && r2 < X.rdpd2 * 1.0001f;
-    if (!(r2 < tf[48 + tt]) && !maybe_dpd) return 0;
-    bool lj = r2 < tf[32 + tt];
-    if (!lj && r2 < tf[48 + tt]) {
+    if (!(r2 < T[F_REACH * 32 + ix])) return 0;
+    bool lj = r2 < T[F_LO * 32 + ix];
+    if (!lj && r2 < T[F_HI * 32 + ix]) {
         // borderline: decide with the FP64 reference arithmetic
+        const int tt = ((ti - 1) & 3) + 4 * ((fwtype(pj.w) - 1) & 3);
         const double4 pa = pos[k], pb = pos[m];
         double ex = mic(pa.x - pb.x, P.L[0], P.invL[0]);
         double ey = mic(pa.y - pb.y, P.L[1], P.invL[1]);
@@ -577,11 +584,11 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
         a.np += 1;
         if (X.fso && r2 < X.r_2_min) r2 = X.r_2_min;
         const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
-        const float l_eps = tf[tt];
-        const float r_61 = tf[16 + tt] * inv_r_2;
+        const float l_eps = T[F_EPS * 32 + ix];
+        const float r_61 = T[F_SIG2 * 32 + ix] * inv_r_2;
         const float r_6 = r_61 * r_61 * r_61;
         const float r_12 = r_6 * r_6;
-        a.e += (double)(l_eps * ((r_12 - r_6) - tf[64 + tt]));
+        a.e += (double)(l_eps * ((r_12 - r_6) - T[F_ESH * 32 + ix]));
         const float rd = l_eps * __builtin_fmaf(6.0f, r_6, -12.0f * r_12) * inv_r_2;
         const float fx = -rd * d.x, fy = -rd * d.y, fz = -rd * d.z;
         a.fx += (double)fx; a.fy += (double)fy; a.fz += (double)fz;
@@ -591,7 +598,9 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
         }
     }
     if (!DPD) return 0;
-    const bool do_dpd = X.dpd_cut ? (ti != 1 && tj != 1 && r2 < X.rdpd2) : lj;
+    // DPD_CUT_OFF: the F_DPD entry is the DPD range^2 for non-head pairs and
+    // -1 for pairs with a head (or a channel without the DPD thermostat)
+    const bool do_dpd = X.dpd_cut ? (r2 < T[F_DPD * 32 + ix]) : lj;
     return do_dpd ? (lj ? 1 : 2) : 0;
 }
 
@@ -599,9 +608,9 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *tf, const
 // pair is within the LJ range, where the switch-on clamp of r^2 applies as in
 // the LJ part.
 template <bool VIR>
-__device__ __forceinline__ void pair32_dpd(const SysDev &P, const float *tf, const Pair32Ctx &X, int ti, int oi,
-                                           float4 pf, float4 vi, float ox, float oy, float oz, float4 pj,
-                                           float4 vj, bool lj, PairAcc &a) {
+__device__ __forceinline__ void pair32_dpd(const float *T, const Pair32Ctx &X, int oi, float4 pf, float4 vi,
+                                           float ox, float oy, float oz, float4 pj, float4 vj, bool lj, int ix,
+                                           PairAcc &a) {
 #pragma clang fp contract(off)
     const Sep32 d = sep32(pf, pj, ox, oy, oz);
     float r2 = d.r2;
@@ -611,8 +620,7 @@ __device__ __forceinline__ void pair32_dpd(const SysDev &P, const float *tf, con
     if (X.dpd_cut) {
         wgt = 1.0f - r2 * inv_sqrt * X.inv_rdpd;
     } else {
-        const int tt = ((ti - 1) & 3) + 4 * ((fwtype(pj.w) - 1) & 3);
-        wgt = 1.0f - r2 * inv_sqrt * tf[80 + tt];
+        wgt = 1.0f - r2 * inv_sqrt * T[F_IRC * 32 + ix];
     }
     const float g = pair_gauss(X.seed, X.t_idx, oi - X.part_base, fworig(pj.w) - X.part_base);
     const float rvx = d.x * inv_sqrt, rvy = d.y * inv_sqrt, rvz = d.z * inv_sqrt;
@@ -643,7 +651,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     double4 *__restrict__ frc = CUR(B.frc);
     const long long *t_dev = B.t_idx;
     int *rebuild = B.rebuild;
-    __shared__ float tf[96];
+    __shared__ float T[F_NF * 32];
     __shared__ float4 off[128];         // cell offset (dx lx, dy ly, dz lz) per list code
     __shared__ double red[BLK];
     __shared__ int dq_m[DPD_Q * BLK];              // deferred DPD pairs: partner slot, [j * BLK + lane]
@@ -651,27 +659,6 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const SysDev &P = sys[s];
     const int S = P.S;
     const float lx = (float)P.l_cell[0], ly = (float)P.l_cell[1], lz = (float)P.l_cell[2];
-    {
-        const int t = threadIdx.x;
-        if (t < 16) {
-            tf[t] = P.epsil_f[t];
-            tf[16 + t] = P.sigma_2f[t];
-            tf[32 + t] = P.range_2f_lo[t];
-            tf[48 + t] = P.range_2f_hi[t];
-            tf[64 + t] = P.e_shift_f[t];
-            tf[80 + t] = P.inv_rc_f[t];
-        }
-        if (t < 128) {
-            const int w = 2 * S + 1;
-            const int dx = t % w - S, dy = (t / w) % w - S, dz = t / (w * w) - S;
-            off[t] = make_float4((float)dx * lx, (float)dy * ly, (float)dz * lz, 0.f);
-        }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
-    if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
-    __syncthreads();
-    const int k = blockIdx.x * BLK + threadIdx.x;
-    const bool active = (k - P.slot_base) < P.n_part;
     Pair32Ctx X;
     X.t_idx = (uint64_t)*t_dev;
     X.r_2_min = 0.0f;
@@ -688,6 +675,38 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // a channel whose thermostat is not DPD flags no pair for a DPD part (as
     // in pair64_ctx: the DPD_CUT_OFF rule with an empty range)
     if (P.thermostat != 1) { X.dpd_cut = 1; X.rdpd2 = -1.0f; }
+    {
+        const int t = threadIdx.x;
+        if (t < 32) {
+            // entry (bead type row, partner type bits): type bits outside 1..4
+            // (padding) never reach
+            const int row = t >> 3, tb = t & 7;
+            const bool valid = tb >= 1 && tb <= 4;
+            const int tt = row + 4 * ((tb - 1) & 3);
+            const bool head_pair = row == 0 || tb == 1;
+            const float hi = P.range_2f_hi[tt];
+            const float dpd2 = (X.dpd_cut && !head_pair) ? X.rdpd2 : -1.0f;
+            const float reach = (DPD && X.dpd_cut && !head_pair) ? fmaxf(hi, X.rdpd2 * 1.0001f) : hi;
+            T[F_REACH * 32 + t] = valid ? reach : -1.0f;
+            T[F_LO * 32 + t] = valid ? P.range_2f_lo[tt] : -1.0f;
+            T[F_HI * 32 + t] = valid ? hi : -1.0f;
+            T[F_EPS * 32 + t] = P.epsil_f[tt];
+            T[F_SIG2 * 32 + t] = P.sigma_2f[tt];
+            T[F_ESH * 32 + t] = P.e_shift_f[tt];
+            T[F_DPD * 32 + t] = valid ? dpd2 : -1.0f;
+            T[F_IRC * 32 + t] = P.inv_rc_f[tt];
+        }
+        if (t < 128) {
+            const int w = 2 * S + 1;
+            const int dx = t % w - S, dy = (t / w) % w - S, dz = t / (w * w) - S;
+            off[t] = make_float4((float)dx * lx, (float)dy * ly, (float)dz * lz, 0.f);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
+    if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
+    __syncthreads();
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    const bool active = (k - P.slot_base) < P.n_part;
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
@@ -696,6 +715,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         const float4 pf = posf[k];
         const int ti = fwtype(pf.w), oi = fworig(pf.w);
         guard(B, ti >= 1 && ti <= 4, G_TYPE, k);
+        const int trow8 = ((ti - 1) & 3) * 8;
         float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
         if (DPD) vi = velf[k];
         int *const myq = dq_m + threadIdx.x;
@@ -706,41 +726,44 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         // is queued, or evaluated in place when the queue is full
         auto visit = [&](int m, int code, float4 pj) {
             const float4 o = off[code];
-            const int f = pair32_lj<DPD, VIR>(P, tf, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, a);
+            const int ix = trow8 | (int)(__float_as_uint(pj.w) & 7u);
+            const int f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, a);
             if (DPD && f) {
                 if (dq < dcap) {
                     myq[dq * BLK] = m;
                     myc[dq * BLK] = (unsigned short)(code | (f == 1 ? 0x80 : 0));
                     ++dq;
                 } else {
-                    pair32_dpd<VIR>(P, tf, X, ti, oi, pf, vi, o.x, o.y, o.z, pj, velf[m], f == 1, a);
+                    pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, pj, velf[m], f == 1, ix, a);
                 }
             }
         };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
-            const unsigned nsl = (unsigned)B.n_slots;
-            for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
-                int ent[NL_UNROLL];
-                float4 pj[NL_UNROLL];
+            // list entries are slots below n_slots (k_nlist takes them from
+            // checked cell ranges); the clamp only keeps a gather in bounds
+            const unsigned last = (unsigned)B.n_slots - 1u;
+            // one 16-byte group of four entries, their four gathers in
+            // flight together; TAIL: the list's last, partial group
+            auto group = [&](int q0, auto tail) {
                 const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
-                ent[0] = e4.x; ent[1] = e4.y; ent[2] = e4.z; ent[3] = e4.w;
-                // a negative entry shifts to a slot of at least 2^24, above
-                // every valid slot, so one unsigned compare is the whole range
-                // check; the speculative gather is clamped and an
-                // out-of-range slot is never visited
+                const int ent[NL_UNROLL] = {e4.x, e4.y, e4.z, e4.w};
+                unsigned mm[NL_UNROLL];
+                float4 pj[NL_UNROLL];
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    const unsigned m = (unsigned)ent[u] >> NL_CODE_BITS;
-                    pj[u] = ld_slot(posf, min(m, nsl - 1u));
+                    mm[u] = min((unsigned)ent[u] >> NL_CODE_BITS, last);
+                    pj[u] = ld_slot(posf, mm[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    const unsigned m = (unsigned)ent[u] >> NL_CODE_BITS;
-                    if (q0 + u < nn && guard(B, m < nsl, G_SLOT, k))
-                        visit((int)m, ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
+                    if (!decltype(tail)::value || q0 + u < nn)
+                        visit((int)mm[u], ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
                 }
-            }
+            };
+            int q0 = 0;
+            for (; q0 + NL_UNROLL <= nn; q0 += NL_UNROLL) group(q0, std::false_type{});
+            if (q0 < nn) group(q0, std::true_type{});
         } else {
             int c = cell_of[k] - P.cell_base;
             if (!guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) c = 0;
@@ -789,7 +812,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 for (int u = 0; u < DPD_UNROLL; ++u) {
                     if (j0 + u < dq) {
                         const float4 o = off[cc[u] & 0x7f];
-                        pair32_dpd<VIR>(P, tf, X, ti, oi, pf, vi, o.x, o.y, o.z, pj[u], vj[u], (cc[u] & 0x80) != 0, a);
+                        const int ix = trow8 | (int)(__float_as_uint(pj[u].w) & 7u);
+                        pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, pj[u], vj[u], (cc[u] & 0x80) != 0, ix, a);
                     }
                 }
             }
