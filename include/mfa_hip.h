@@ -147,7 +147,9 @@ int  mfa_hip_new_dpd_fd(mfa_ctx *ctx);
  * (integrate+skin check, [gated cell rebuild], pair, bonded+wall+kick).
  * use_graph=1 replays captured graphs of 8 steps and of 1 step.  The force
  * that mfa_hip_get_state returns afterwards is the total force of the last
- * step (the steps before it do not write it back). */
+ * step, and the energies and pair count that mfa_hip_get_energies and
+ * mfa_hip_get_pair_count return are the last step's (the steps before it
+ * neither write the force back nor sum the energies). */
 int  mfa_hip_md_steps(mfa_ctx *ctx, int32_t n_steps, int32_t use_graph);
 /* Captures the graphs md_steps(use_graph=1) replays, without running a step
  * (a timed region then pays no capture); a no-op while timing. */

@@ -379,26 +379,27 @@ static void enqueue_vv1(mfa_ctx *c) {
     else LAUNCH((k_vv1<false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
 }
 
-static void enqueue_pair(mfa_ctx *c) {
+// fused: inside md_steps (energies and the pair count on the call's last step only)
+static void enqueue_pair(mfa_ctx *c, int fused) {
     Timer t(c, K_PAIR);
     bool dpd = false;
     for (auto &p : c->hp) dpd = dpd || p.thermostat == 1;
     const int nb = c->n_blocks;
     if (c->prec == MFA_PREC_MIXED) {
         if (dpd) {
-            if (c->virial) LAUNCH((k_pair32<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else LAUNCH((k_pair32<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair32<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            else LAUNCH((k_pair32<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
         } else {
-            if (c->virial) LAUNCH((k_pair32<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else LAUNCH((k_pair32<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair32<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            else LAUNCH((k_pair32<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
         }
     } else {
         if (dpd) {
-            if (c->virial) LAUNCH((k_pair64<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else LAUNCH((k_pair64<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair64<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            else LAUNCH((k_pair64<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
         } else {
-            if (c->virial) LAUNCH((k_pair64<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
-            else LAUNCH((k_pair64<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, 1);
+            if (c->virial) LAUNCH((k_pair64<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            else LAUNCH((k_pair64<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
         }
     }
 }
@@ -423,7 +424,7 @@ static void enqueue_dpd_vv(mfa_ctx *c, int all, int fused) {
 static void enqueue_step(mfa_ctx *c) {
     enqueue_vv1(c);
     enqueue_rebuild(c, 0);
-    enqueue_pair(c);
+    enqueue_pair(c, 1);
     bool bend = false, dpd_vv = false;
     for (const SysDev &d : c->hs) {
         bend = bend || d.bending || d.bending_melt;
@@ -811,7 +812,7 @@ int mfa_hip_fluid_fluid(mfa_ctx *c) {
     if (int rc = check_ctx(c)) return rc;
     enqueue_rebuild(c, 1);    // pending rebuild after set_state / verlet_positions (gated)
     if (c->prec == MFA_PREC_MIXED) LAUNCH(k_velf_sync, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B);
-    enqueue_pair(c);
+    enqueue_pair(c, 0);
     HIPCHK(hipGetLastError());
     return 0;
 }

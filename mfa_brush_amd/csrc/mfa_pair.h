@@ -132,7 +132,7 @@ __device__ __forceinline__ Pair64Ctx pair64_ctx(const SysDev &P, long long t_idx
 // it (DPD_CUT_OFF only).
 template <bool DPD, bool VIR>
 __device__ __forceinline__ int pair64_lj(const Pair64Ctx &C, const double *tab, double xi, double yi, double zi,
-                                         int ti, double4 pj, PairAcc &a) {
+                                         int ti, double4 pj, bool calc_e, PairAcc &a) {
 #pragma clang fp contract(off)
     const int tj = wtype(pj.w);
     const int tt = ((ti - 1) & 3) + 4 * ((tj - 1) & 3);
@@ -142,14 +142,16 @@ __device__ __forceinline__ int pair64_lj(const Pair64Ctx &C, const double *tab, 
     double r_2 = r2_exact(dx, dy, dz);
     const bool lj = r_2 < tab[32 + tt];
     if (lj) {
-        a.np += 1;
         if (C.fso && r_2 < C.r_2_min) r_2 = C.r_2_min;
         const double inv_r_2 = 1.0 / r_2;
         const double l_eps = tab[tt];
         const double r_61 = tab[16 + tt] * inv_r_2;
         const double r_6 = r_61 * r_61 * r_61;
         const double r_12 = r_6 * r_6;
-        a.e += l_eps * ((r_12 - r_6) - tab[48 + tt]);
+        if (calc_e) {
+            a.np += 1;
+            a.e += l_eps * ((r_12 - r_6) - tab[48 + tt]);
+        }
         const double rd = l_eps * (-12.0 * r_12 + 6.0 * r_6) * inv_r_2;
         const double fx = -rd * dx, fy = -rd * dy, fz = -rd * dz;
         a.fx += fx; a.fy += fy; a.fz += fz;
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
 template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
-                                               double *__restrict__ partial, int assign) {
+                                               double *__restrict__ partial, int fused) {
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     MFA_CUR(B, s);
     const double4 *__restrict__ pos = CUR(B.pos);
@@ -413,6 +415,10 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         if (r_time < P.r_2_min_time) r_2_min = (1.0 - r_time / P.r_2_min_time) * P.r_2_min_init;
     }
     const Pair64Ctx C = pair64_ctx(P, t_idx, r_2_min);
+    // the fused step sums the energy and counts the pairs only on the step
+    // whose force it stores (the last of an md_steps call): only that
+    // step's energies can be read
+    const bool calc_e = !fused || __builtin_amdgcn_readfirstlane(*B.store_frc) != 0;
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
@@ -429,7 +435,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         // LJ part of candidate m (position pj); a DPD pair is queued, or
         // evaluated in place when the queue is full
         auto visit = [&](int m, double4 pj) {
-            const int f = pair64_lj<DPD, VIR>(C, tab, pi.x, pi.y, pi.z, ti, pj, a);
+            const int f = pair64_lj<DPD, VIR>(C, tab, pi.x, pi.y, pi.z, ti, pj, calc_e, a);
             if (DPD && f) {
                 if (dq < dcap) {
                     myq[dq * BLK] = (unsigned)m | (f == 1 ? 0x80000000u : 0u);
@@ -505,12 +511,12 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                 }
             }
         }
-        double4 f = assign ? make_double4(0.0, 0.0, 0.0, 0.0) : frc[k];
-        f.x += a.fx; f.y += a.fy; f.z += a.fz;
+        double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, (uint64_t)t_idx, oi - P.part_base, ti, vel[k], f);
         frc[k] = f;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
+    if (!calc_e) return;
     double e = block_sum(a.e, red);
     double np = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = e; out[C_NPAIR] = np; }
@@ -565,7 +571,8 @@ enum T32Field { F_REACH = 0, F_LO, F_HI, F_EPS, F_SIG2, F_ESH, F_DPD, F_IRC, F_N
 template <bool DPD, bool VIR>
 __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const Pair32Ctx &X,
                                          const double4 *__restrict__ pos, int k, int ti, float4 pf,
-                                         float ox, float oy, float oz, float4 pj, int m, int ix, PairAcc &a) {
+                                         float ox, float oy, float oz, float4 pj, int m, int ix, bool calc_e,
+                                         PairAcc &a) {
 #pragma clang fp contract(off)
     const Sep32 d = sep32(pf, pj, ox, oy, oz);
     float r2 = d.r2;
@@ -581,14 +588,16 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const 
         lj = r2_exact(ex, ey, ez) < P.range_2[tt];
     }
     if (lj) {
-        a.np += 1;
         if (X.fso && r2 < X.r_2_min) r2 = X.r_2_min;
         const float inv_r_2 = __builtin_amdgcn_rcpf(r2);
         const float l_eps = T[F_EPS * 32 + ix];
         const float r_61 = T[F_SIG2 * 32 + ix] * inv_r_2;
         const float r_6 = r_61 * r_61 * r_61;
         const float r_12 = r_6 * r_6;
-        a.e += (double)(l_eps * ((r_12 - r_6) - T[F_ESH * 32 + ix]));
+        if (calc_e) {
+            a.np += 1;
+            a.e += (double)(l_eps * ((r_12 - r_6) - T[F_ESH * 32 + ix]));
+        }
         const float rd = l_eps * __builtin_fmaf(6.0f, r_6, -12.0f * r_12) * inv_r_2;
         const float fx = -rd * d.x, fy = -rd * d.y, fz = -rd * d.z;
         a.fx += (double)fx; a.fy += (double)fy; a.fz += (double)fz;
@@ -641,7 +650,7 @@ __device__ __forceinline__ void pair32_dpd(const float *T, const Pair32Ctx &X, i
 template <bool DPD, bool VIR>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
                                                Bufs B, const int2 *__restrict__ crange,
-                                               double *__restrict__ partial, int assign) {
+                                               double *__restrict__ partial, int fused) {
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     MFA_CUR(B, s);
     const double4 *__restrict__ pos = CUR(B.pos);
@@ -704,6 +713,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
     if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
+    // energy and pair count only on the step whose force is stored (k_pair64)
+    const bool calc_e = !fused || __builtin_amdgcn_readfirstlane(*B.store_frc) != 0;
     __syncthreads();
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
@@ -727,7 +738,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         auto visit = [&](int m, int code, float4 pj) {
             const float4 o = off[code];
             const int ix = trow8 | (int)(__float_as_uint(pj.w) & 7u);
-            const int f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, a);
+            const int f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, calc_e, a);
             if (DPD && f) {
                 if (dq < dcap) {
                     myq[dq * BLK] = m;
@@ -818,12 +829,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 }
             }
         }
-        double4 f = assign ? make_double4(0.0, 0.0, 0.0, 0.0) : frc[k];
-        f.x += a.fx; f.y += a.fy; f.z += a.fz;
+        double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, X.t_idx, oi - P.part_base, ti, CUR(B.vel)[k], f);
         frc[k] = f;
     }
     double *out = partial + (size_t)blockIdx.x * NPART;
+    if (!calc_e) return;
     double es = block_sum(a.e, red);
     double nps = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }

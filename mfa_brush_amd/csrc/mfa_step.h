@@ -309,16 +309,20 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         f4.x = F.x; f4.y = F.y; f4.z = F.z;
         if (store_f) CUR(B.frc)[k] = f4;
     }
+    // the fused step sums its energies only on the step whose force it
+    // stores (the last of an md_steps call), the only one a caller can read
     double *out = partial + (size_t)blockIdx.x * NPART;
-    if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
-    if (MODE & (M_FENE | M_FUSED)) { double t = block_sum(e_fene, red); if (threadIdx.x == 0) out[C_VINTRA] = t; }
-    if ((MODE & M_BEND) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
-    if ((MODE & M_BENDM) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
-    if (MODE & (M_VV2 | M_FUSED)) {
-        double t = block_sum(ke, red);
-        if (threadIdx.x == 0) out[C_KE] = t;
-        if (blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
+    if (store_f) {
+        if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
+        if (MODE & (M_FENE | M_FUSED)) { double t = block_sum(e_fene, red); if (threadIdx.x == 0) out[C_VINTRA] = t; }
+        if ((MODE & M_BEND) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
+        if ((MODE & M_BENDM) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
+        if (MODE & (M_VV2 | M_FUSED)) {
+            double t = block_sum(ke, red);
+            if (threadIdx.x == 0) out[C_KE] = t;
+        }
     }
+    if ((MODE & (M_VV2 | M_FUSED)) && blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
 }
 
 // ---------------------------------------------------------------------------
