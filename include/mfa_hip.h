@@ -238,6 +238,13 @@ int  mfa_hip_chain_observables(mfa_ctx *ctx, int32_t isys, double *out);
  * reference; a tuning and test knob of this implementation. */
 int  mfa_hip_set_dpd_queue(mfa_ctx *ctx, int32_t depth);
 
+/* Launch shape of the mixed pair kernel: 0 (default) one block per tile of
+ * 256 slots; > 0 a persistent launch of that many blocks that take the tiles
+ * from a device work counter (fewer, longer-lived blocks; the order of work
+ * inside a tile and the results are the same bits).  No counterpart in the
+ * reference; a tuning knob (mfa_brush_amd/autotune.py). */
+int  mfa_hip_set_pair_grid(mfa_ctx *ctx, int32_t blocks);
+
 const char *mfa_hip_last_error(void);
 int  mfa_hip_version(void);
 

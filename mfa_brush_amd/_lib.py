@@ -92,6 +92,7 @@ SIGNATURES = {
     "mfa_hip_set_neighbor_list": [_vp, _i32],
     "mfa_hip_get_neighbor_list": [_vp, _vp],
     "mfa_hip_set_dpd_queue": [_vp, _i32],
+    "mfa_hip_set_pair_grid": [_vp, _i32],
     "mfa_hip_chain_observables": [_vp, _i32, _vp],
     "mfa_hip_get_pair_count": [_vp, _i32, _vp],
     "mfa_hip_last_error": [],

@@ -69,6 +69,8 @@ struct mfa_ctx {
     int *d_bin_cell = nullptr, *d_bin_rank = nullptr, *d_perm = nullptr, *d_key_tmp = nullptr, *d_bin_key = nullptr;
     double *d_partial = nullptr, *d_energies = nullptr, *d_fheads = nullptr;
     double4 *d_dfd = nullptr;      // DPD-VV: change of each bead's dissipative force (slot order)
+    int *d_tiles = nullptr;        // persistent mixed pair kernel: work counter, ticket counter
+    int pair_grid = 0;             // 0: one block per tile; > 0: persistent launch of this many blocks
     int *d_rebuild_count = nullptr, *d_permute_done = nullptr;
     double *d_obs = nullptr;       // block partials of the chain observables (2 x channel blocks x 8)
     double *d_stage = nullptr;
@@ -195,7 +197,7 @@ static void free_all(mfa_ctx *c) {
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
     F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_obs); F(c->d_stage); F(c->d_istage);
-    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2); F(c->d_dfd);
+    F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2); F(c->d_dfd); F(c->d_tiles);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -386,13 +388,23 @@ static void enqueue_pair(mfa_ctx *c, int fused) {
     for (auto &p : c->hp) dpd = dpd || p.thermostat == 1;
     const int nb = c->n_blocks;
     if (c->prec == MFA_PREC_MIXED) {
+        // persistent launch: pair_grid blocks take the nb tiles from a counter
+        const bool pers = c->pair_grid > 0 && c->pair_grid < nb;
+        const int g = pers ? c->pair_grid : nb;
+        int *tl = pers ? c->d_tiles : nullptr;
+#define PAIR32(D, V)                                                                                     \
+    do {                                                                                                 \
+        if (pers) LAUNCH((k_pair32<D, V, true>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange,     \
+                         c->d_partial, fused, tl, nb);                                                   \
+        else LAUNCH((k_pair32<D, V, false>), g, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange,         \
+                    c->d_partial, fused, tl, nb);                                                        \
+    } while (0)
         if (dpd) {
-            if (c->virial) LAUNCH((k_pair32<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
-            else LAUNCH((k_pair32<true, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            if (c->virial) PAIR32(true, true); else PAIR32(true, false);
         } else {
-            if (c->virial) LAUNCH((k_pair32<false, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
-            else LAUNCH((k_pair32<false, false>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
+            if (c->virial) PAIR32(false, true); else PAIR32(false, false);
         }
+#undef PAIR32
     } else {
         if (dpd) {
             if (c->virial) LAUNCH((k_pair64<true, true>), nb, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_crange, c->d_partial, fused);
@@ -682,6 +694,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     }
     ALLOC(c->d_stage, 3 * (size_t)c->N_tot); ALLOC(c->d_istage, c->N_tot);
     ALLOC(c->d_dfd, c->S_tot);
+    ALLOC(c->d_tiles, 2);
 #undef ALLOC
     if ((rc = alloc_list(c))) { free_all(c); delete c; return rc; }
     // the zero fills of dalloc ran on the null stream: finish them before
@@ -1194,6 +1207,15 @@ int mfa_hip_set_dpd_queue(mfa_ctx *c, int32_t depth) {
     if (depth < 0 || depth > DPD_Q) return fail(MFA_E_ARG, "set_dpd_queue: depth must be 0..8");
     HIPCHK(hipStreamSynchronize(c->st));
     c->B.dpd_queue = depth;
+    invalidate_graph(c);
+    return 0;
+}
+
+int mfa_hip_set_pair_grid(mfa_ctx *c, int32_t blocks) {
+    if (int rc = check_ctx(c, false)) return rc;
+    if (blocks < 0) return fail(MFA_E_ARG, "set_pair_grid: blocks must be >= 0");
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->pair_grid = blocks;
     invalidate_graph(c);
     return 0;
 }

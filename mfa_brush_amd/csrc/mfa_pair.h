@@ -643,15 +643,13 @@ __device__ __forceinline__ void pair32_dpd(const float *T, const Pair32Ctx &X, i
     }
 }
 
-// Six waves per SIMD: caps the kernel at 80 VGPRs (91 without the hint, five
-// waves) for a 12-byte spill; measured 5-9 % faster on the pair step at 0.3-5M
-// beads.  Seven waves (72 VGPRs, 40-byte spill) is 60 % slower, and five waves
-// forced on k_pair64 (96 VGPRs) is slower than its natural four.
+// One tile of 256 slots (the tile-th block's slots) of the mixed pair kernel.
 template <bool DPD, bool VIR>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pair32(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
-                                               Bufs B, const int2 *__restrict__ crange,
-                                               double *__restrict__ partial, int fused) {
-    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+__device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__restrict__ sys,
+                                            const int *__restrict__ blk_sys, const Bufs &B,
+                                            const int2 *__restrict__ crange, double *__restrict__ partial,
+                                            int fused) {
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[tile]);
     MFA_CUR(B, s);
     const double4 *__restrict__ pos = CUR(B.pos);
     const float4 *__restrict__ posf = CUR(B.posf);
@@ -711,12 +709,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             off[t] = make_float4((float)dx * lx, (float)dy * ly, (float)dz * lz, 0.f);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *rebuild = 0;
-    if ((int)blockIdx.x == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
+    if (tile == 0 && threadIdx.x == 0) *rebuild = 0;
+    if (tile == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
     // energy and pair count only on the step whose force is stored (k_pair64)
     const bool calc_e = !fused || __builtin_amdgcn_readfirstlane(*B.store_frc) != 0;
     __syncthreads();
-    const int k = blockIdx.x * BLK + threadIdx.x;
+    const int k = tile * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
@@ -833,7 +831,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         if (P.thermostat == 2) lgv_bead(P, X.t_idx, oi - P.part_base, ti, CUR(B.vel)[k], f);
         frc[k] = f;
     }
-    double *out = partial + (size_t)blockIdx.x * NPART;
+    double *out = partial + (size_t)tile * NPART;
     if (!calc_e) return;
     double es = block_sum(a.e, red);
     double nps = block_sum((double)a.np, red);
@@ -841,6 +839,41 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     for (int q = 0; q < 6; ++q) {       // zero when the virial is off
         double wq = VIR ? block_sum(a.w[q], red) : 0.0;
         if (threadIdx.x == 0) out[C_WXX + q] = wq;
+    }
+}
+
+// Six waves per SIMD: caps the kernel at 80 VGPRs (91 without the hint, five
+// waves) for a 12-byte spill; measured 5-9 % faster on the pair step at 0.3-5M
+// beads.  Seven waves (72 VGPRs, 40-byte spill) is 60 % slower, and five waves
+// forced on k_pair64 (96 VGPRs) is slower than its natural four.
+//
+// PERSIST false: one tile per block.  PERSIST true: a persistent launch: the
+// blocks take tiles from the work counter tiles[0] until it passes n_tiles
+// (every block's last take does, so every block leaves the loop); the last
+// block to leave resets tiles[0] and tiles[1] (its ticket counter) to 0 for
+// the next launch.
+template <bool DPD, bool VIR, bool PERSIST>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PERSIST ? 5 : 6, 8))) void k_pair32(
+    const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B, const int2 *__restrict__ crange,
+    double *__restrict__ partial, int fused, int *tiles, int n_tiles) {
+    if constexpr (!PERSIST) {
+        pair32_tile<DPD, VIR>(blockIdx.x, sys, blk_sys, B, crange, partial, fused);
+        return;
+    }
+    __shared__ int next;
+    for (;;) {
+        // every thread is past its previous tile (and its reads of the LDS
+        // tables) once it reaches this barrier
+        __syncthreads();
+        if (threadIdx.x == 0) next = atomicAdd(&tiles[0], 1);
+        __syncthreads();
+        const int tile = next;
+        if (tile >= n_tiles) break;
+        pair32_tile<DPD, VIR>(tile, sys, blk_sys, B, crange, partial, fused);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&tiles[1], 1) == (int)gridDim.x - 1) {
+        tiles[0] = 0;
+        tiles[1] = 0;
     }
 }
 

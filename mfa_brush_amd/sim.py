@@ -262,6 +262,10 @@ class Simulation:
         """DPD pairs per bead whose DPD part is deferred to a second pass (0..8)."""
         self._c(self.lib.mfa_hip_set_dpd_queue(self.h, int(depth)), "set_dpd_queue")
 
+    def set_pair_grid(self, blocks: int = 0):
+        """Mixed pair kernel launch: 0 one block per tile, > 0 persistent blocks."""
+        self._c(self.lib.mfa_hip_set_pair_grid(self.h, int(blocks)), "set_pair_grid")
+
     def neighbor_list(self) -> int:
         n = C.c_int32()
         self._c(self.lib.mfa_hip_get_neighbor_list(self.h, C.byref(n)), "get_neighbor_list")

@@ -229,3 +229,30 @@ def test_dpd_vv_thermostat_holds_kt():
         assert s.flags()["nonfinite"] == 0
     kt = 2.0 * np.mean(kes) / (3 * (p.n_part - p.n_chain))
     assert abs(kt - 1.0) < 0.05, kt
+
+
+@pytest.mark.parametrize("blocks", [7, 64])
+def test_persistent_pair_launch_same_bits(blocks):
+    """The persistent launch of the mixed pair kernel (a few blocks taking
+    tiles from a device counter, which they reset for the next launch) gives
+    the bits of one block per tile, over graph-replayed steps with rebuilds,
+    on one channel and on a batch of uneven replicas."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.3)
+    reps = [configs.channel_scaled(n, n_mon=8, lz=14.0, seed=300 + n, skin=0.3) for n in (6, 9, 7)]
+    for ps, states in (([p], [(t, r0, v)]), ([r[0] for r in reps], [(r[1], r[2], r[3]) for r in reps])):
+        outs = []
+        for grid in (0, blocks):
+            with Simulation(ps, precision="mixed") as s:
+                s.set_pair_grid(grid)
+                for i, (tt, rr, vv) in enumerate(states):
+                    s.set_state(rr, vv, tt, isys=i)
+                s.md_steps(30, graph=True)
+                outs.append([s.get_state(i) for i in range(len(ps))] + [s.energies(i)["v_fluid_fluid"] for i in range(len(ps))])
+                assert s.flags()["rebuilds"] > 1
+        for a, b in zip(outs[0], outs[1]):
+            if isinstance(a, dict):
+                for k in ("r0", "v", "a", "force"):
+                    assert np.array_equal(a[k], b[k]), k
+            else:
+                assert a == b
