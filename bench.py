@@ -34,7 +34,7 @@ PEAK_FP64 = 78.6e12          # MI355X vector FP64
 HBM_PEAK = 8.0e12
 
 
-def run_gpu(precision, steps, warmup, ps, states, nlist=-1):
+def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None):
     """Warm-up steps (graph replay), then the timed region: `steps` steps
     launched in order with a pair of HIP events around each pair-kernel
     launch on the library's stream (so the pair kernel's average launch
@@ -46,7 +46,11 @@ def run_gpu(precision, steps, warmup, ps, states, nlist=-1):
     batched) context."""
     from mfa_brush_amd.sim import Simulation
     s = Simulation(ps if len(ps) > 1 else ps[0], precision=precision)
-    s.set_neighbor_list(nlist)
+    if choice is not None:
+        from mfa_brush_amd import autotune
+        autotune.apply(s, choice, s.neighbor_list())
+    else:
+        s.set_neighbor_list(nlist)
     for i, (t, r0, v) in enumerate(states):
         s.set_state(r0, v, t, isys=i)
     s.md_steps(warmup, graph=True)
@@ -161,7 +165,9 @@ def main():
     ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
                     help="c2: 1M-bead brush channel (the metric); c3: 1.2M-bead melt with bending and "
                          "Poiseuille flow; c4: 256 batched replicas")
-    ap.add_argument("--autotune", action="store_true", help="pick the skin by timing short runs first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick skin, list capacity, pair-kernel launch shape and DPD queue depth by timing short "
+                         "runs first (coordinate descent, mfa_brush_amd/autotune.py)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -184,7 +190,7 @@ def main():
         args.skin = tuned["skin"]
 
     nlist = 0 if args.no_list else -1
-    mixed = run_gpu("mixed", args.steps, args.warmup, ps, states, nlist)
+    mixed = run_gpu("mixed", args.steps, args.warmup, ps, states, nlist, tuned)
     print("mixed: %.3f s for %d steps" % (mixed["seconds"], args.steps), file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:

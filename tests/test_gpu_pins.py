@@ -256,3 +256,28 @@ def test_persistent_pair_launch_same_bits(blocks):
                     assert np.array_equal(a[k], b[k]), k
             else:
                 assert a == b
+
+
+def test_autotuner_choices_and_cache(tmp_path):
+    """The autotuner times every candidate of each knob in turn, returns the
+    fastest setting, caches it by workload signature, and the setting it
+    returns runs the same pair set (same positions within the FP64 summation
+    order of the deferred DPD parts)."""
+    from mfa_brush_amd import autotune
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=True, rc_dpd=1.0)
+    cache = str(tmp_path / "tune.json")
+    r = autotune.tune([p], [(t, r0, v)], "mixed", skins=(0.3, 0.4), grids=(0, 64), queues=(8, 4),
+                      lists=(-1, 1.25), steps=8, warmup=4, cache=cache)
+    assert len(r["timings"]) == 8
+    assert r["skin"] in (0.3, 0.4) and r["pair_grid"] in (0, 64) and r["dpd_queue"] in (8, 4)
+    assert r["ms_per_step"] == min(x["ms_per_step"] for x in r["timings"][-2:])
+    again = autotune.tune([p], [(t, r0, v)], "mixed", cache=cache)        # from the cache, no timing
+    assert again == r
+    outs = []
+    for choice in (autotune.DEFAULT, r):
+        with Simulation(p.replace(skin=choice["skin"]), precision="mixed") as s:
+            autotune.apply(s, choice, s.neighbor_list())
+            s.set_state(r0, v, t)
+            s.md_steps(10, graph=True)
+            outs.append(s.get_state()["r0"])
+    assert np.abs(outs[0] - outs[1]).max() < 1e-9
