@@ -32,6 +32,17 @@ FLOPS_PER_PAIR = 79          # LJ 41 + DPD 38 per interacting pair (thermostat o
 PEAK_FP32 = 157.3e12         # MI355X vector FP32
 PEAK_FP64 = 78.6e12          # MI355X vector FP64
 HBM_PEAK = 8.0e12
+# The oracle's OpenMP restatement runs this many times faster than the
+# reference's own routines (flang -O3, OpenMP fluid_fluid, serial
+# integrator) on identical inputs and thread counts: measured by
+# oracle/calibrate.py on an 8-core Xeon host (33,178-bead channel, 8 threads:
+# 1.08e6 vs 4.11e5 particle-steps/s; 1 thread: 1.48x; the 4096-bead channel:
+# 2.59x at 8 threads, 1.42x at 1).  reference_equivalent_value divides the
+# port's rate by it.
+PORT_OVER_REFERENCE = 2.63
+CALIBRATION = ("oracle/calibrate.py: port/reference = 2.63 at 8 threads (33k beads), 1.48 at 1 thread; the "
+               "reference's serial integrator weighs more at higher thread counts, so the reference-"
+               "equivalent figure is an upper bound on the reference's own rate")
 
 
 def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None):
@@ -143,9 +154,13 @@ def cpu_baseline(ps, states, n_steps, max_channels=16):
         o.md_steps(r0, v, np.zeros_like(r0), t, 0, n_steps, noise=1)
         dt += time.perf_counter() - t0
         beads += p.n_part
-    return dict(value=beads * n_steps / dt, unit="particle-steps/s", cores=threads, kind="port",
+    value = beads * n_steps / dt
+    return dict(value=value, unit="particle-steps/s", cores=threads, kind="port",
                 sample="%d steps of %d channel(s) of the workload, %d beads (%.1f s)" %
-                       (n_steps, min(len(ps), max_channels), beads, dt))
+                       (n_steps, min(len(ps), max_channels), beads, dt),
+                port_over_reference=PORT_OVER_REFERENCE,
+                reference_equivalent_value=value / PORT_OVER_REFERENCE,
+                calibration=CALIBRATION)
 
 
 def main():

@@ -10,7 +10,7 @@ REF    ?= /root/reference
 HERE   := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
 OUT    := $(HERE)_ref
 FLANG  ?= /opt/rocm/llvm/bin/flang
-FFLAGS := -cpp -O2 -fPIC -I$(REF)
+FFLAGS := -cpp -O3 -fPIC -I$(REF)
 UNITS  := mfa_common ziggurat par_ziggurat util functions cell_list fluid_fluid dpd_forces intra_molec \
           fluid_wall wall_wall bending constant_force verlet_positions verlet_velocities
 SRCS   := $(addprefix $(REF)/,$(addsuffix .f90,$(UNITS))) $(HERE)ref_harness.f90
