@@ -2,7 +2,7 @@
 
 Needs oracle/_ref (built by `make -f oracle/ref.mk` from the reference
 sources).  The configuration is tests/test_oracle_vs_reference.ref_system
-(n_side=6: 1,152 beads); the vectors are what the reference's fluid_fluid
+(n_side=6: 922 beads); the vectors are what the reference's fluid_fluid
 (LJ + dissipative DPD, sig=0), intra_molec, bending, fluid_wall and 10 steps
 of the md_main loop produce on it.
 """
