@@ -433,18 +433,49 @@ static void enqueue_dpd_vv(mfa_ctx *c, int all, int fused) {
     LAUNCH(k_dpd_vv_apply, c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, (const double4 *)c->d_dfd, all, fused);
 }
 
+static bool any_bending(const mfa_ctx *c) {
+    for (const SysDev &d : c->hs)
+        if (d.bending || d.bending_melt) return true;
+    return false;
+}
+
+// The fused step runs with the fused integrator (k_integrate: the second
+// half kick of a step and the first half of the next one) unless a channel
+// re-evaluates its dissipative forces between the two (DPD-VV).
+static bool fused_integrator(const mfa_ctx *c) {
+    for (const SysDev &d : c->hs)
+        if (d.dpd_vv && d.thermostat == 1) return false;
+    return true;
+}
+
+// One fused step.  With the fused integrator: [gated rebuild], pair forces
+// with the walls and bonds in the pair kernel's epilogue, [bending],
+// k_integrate (which also starts the next step, so md_steps enqueues k_vv1
+// once, before its first step).  Otherwise: k_vv1, [gated rebuild], pair,
+// k_bonded (walls, bonds, bending, body force, second kick), DPD-VV.
 static void enqueue_step(mfa_ctx *c) {
+    if (fused_integrator(c)) {
+        enqueue_rebuild(c, 0);
+        enqueue_pair(c, 2);
+        Timer t(c, K_BONDED);
+        if (any_bending(c)) {
+            if (c->prec == MFA_PREC_MIXED)
+                LAUNCH((k_bonded<M_BENDF, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+            else
+                LAUNCH((k_bonded<M_BENDF, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        }
+        if (c->prec == MFA_PREC_MIXED)
+            LAUNCH((k_integrate<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        else
+            LAUNCH((k_integrate<false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        return;
+    }
     enqueue_vv1(c);
     enqueue_rebuild(c, 0);
     enqueue_pair(c, 1);
-    bool bend = false, dpd_vv = false;
-    for (const SysDev &d : c->hs) {
-        bend = bend || d.bending || d.bending_melt;
-        dpd_vv = dpd_vv || (d.dpd_vv && d.thermostat == 1);
-    }
-    if (bend) enqueue_bonded<M_FUSED>(c);
+    if (any_bending(c)) enqueue_bonded<M_FUSED>(c);
     else enqueue_bonded<M_FUSED | M_NOBEND>(c);
-    if (dpd_vv) enqueue_dpd_vv(c, 0, 1);
+    enqueue_dpd_vv(c, 0, 1);
 }
 
 // Reads the device guard word (one sync): a nonzero code means a kernel met
@@ -899,7 +930,10 @@ int mfa_hip_md_steps(mfa_ctx *c, int32_t n_steps, int32_t use_graph) {
     if (int rc = check_ctx(c)) return rc;
     if (n_steps < 0) return fail(MFA_E_ARG, "md_steps: negative step count");
     ensure_cells(c);
-    if (n_steps > 0) hipLaunchKernelGGL(k_set_t_end, dim3(1), dim3(1), 0, c->st, c->B, (int)n_steps);
+    if (n_steps > 0) {
+        hipLaunchKernelGGL(k_set_t_end, dim3(1), dim3(1), 0, c->st, c->B, (int)n_steps);
+        if (fused_integrator(c)) enqueue_vv1(c);      // the first step's first half
+    }
     if (!use_graph || c->timing_mask) {
         for (int i = 0; i < n_steps; ++i) enqueue_step(c);
         HIPCHK(hipGetLastError());

@@ -34,6 +34,7 @@
 #pragma once
 #include <type_traits>
 
+#include "mfa_bonded.h"
 #include "mfa_kernels.h"
 
 namespace mfa {
@@ -383,6 +384,44 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
     if (n > cap) atomicAdd(&B.flags[4], 1);
 }
 
+// The fused step's bonded epilogue of the pair kernels (mode 2): the 9-3
+// walls and the FENE bonds of bead k (FP64 position ri, global index g, type
+// ti) added to its pair force f in md_main.f90's order (fluid_wall :132,
+// intra_molec :142), energies to e_wall and e_fene.
+__device__ __forceinline__ void bonded_epilogue(const SysDev &P, const Bufs &B, size_t cur_off, const double4 &ri,
+                                                int ti, double4 &f, double &e_wall, double &e_fene) {
+    const int g = worig(ri.w);
+    const int o = g - P.part_base;
+    const bool brush = o < P.pd;
+    const int nm = brush ? P.n_mon : P.n_mon_d;
+    const int ol = brush ? o : o - P.pd;
+    const int m = nm > 0 ? ol % nm : 0;
+    double3 F = make_double3(f.x, f.y, f.z);
+    wall_bead(P, ri, ti, F, e_wall);
+    fene_bead(P, B, cur_off, ri, g, m, nm, F, e_fene);
+    f.x = F.x; f.y = F.y; f.z = F.z;
+}
+
+// Energy partials of a pair-kernel block: the pair energy and count, the
+// virial (zero when off) and, in mode 2, the walls and bonds the epilogue
+// added (bending columns zeroed; the bending kernel, if any, writes them).
+template <bool VIR>
+__device__ __forceinline__ void pair_partials(double *out, const PairAcc &a, int mode, double e_wall, double e_fene,
+                                              double *red) {
+    double es = block_sum(a.e, red);
+    double nps = block_sum((double)a.np, red);
+    if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }
+    for (int q = 0; q < 6; ++q) {       // zero when the virial is off
+        double wq = VIR ? block_sum(a.w[q], red) : 0.0;
+        if (threadIdx.x == 0) out[C_WXX + q] = wq;
+    }
+    if (mode == 2) {
+        double ew = block_sum(e_wall, red);
+        double ef = block_sum(e_fene, red);
+        if (threadIdx.x == 0) { out[C_VWALL] = ew; out[C_VINTRA] = ef; out[C_VBEND] = 0.0; out[C_VBENDM] = 0.0; }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 pair kernel.
 // ---------------------------------------------------------------------------
@@ -415,14 +454,17 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         if (r_time < P.r_2_min_time) r_2_min = (1.0 - r_time / P.r_2_min_time) * P.r_2_min_init;
     }
     const Pair64Ctx C = pair64_ctx(P, t_idx, r_2_min);
-    // the fused step sums the energy and counts the pairs only on the step
-    // whose force it stores (the last of an md_steps call): only that
-    // step's energies can be read
-    const bool calc_e = !fused || __builtin_amdgcn_readfirstlane(*B.store_frc) != 0;
+    // the fused step (mode 1, 2) sums the energy and counts the pairs only on
+    // the last step of an md_steps call, the one whose force it stores and
+    // whose energies a caller can read; block 0 publishes that decision for
+    // the step's later kernels
+    const bool calc_e = fused == 0 || *B.t_idx >= *B.t_end;
+    if (fused != 0 && blockIdx.x == 0 && threadIdx.x == 0) *B.store_frc = calc_e ? 1 : 0;
     PairAcc a;
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
     a.np = 0;
+    double e_wall = 0.0, e_fene = 0.0;
     if (active) {
         const double4 pi = pos[k];
         const int ti = wtype(pi.w), oi = worig(pi.w);
@@ -513,17 +555,11 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         }
         double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, (uint64_t)t_idx, oi - P.part_base, ti, vel[k], f);
+        if (fused == 2) bonded_epilogue(P, B, cur_off, pi, ti, f, e_wall, e_fene);
         frc[k] = f;
     }
-    double *out = partial + (size_t)blockIdx.x * NPART;
     if (!calc_e) return;
-    double e = block_sum(a.e, red);
-    double np = block_sum((double)a.np, red);
-    if (threadIdx.x == 0) { out[C_VFF] = e; out[C_NPAIR] = np; }
-    for (int q = 0; q < 6; ++q) {       // zero when the virial is off
-        double wq = VIR ? block_sum(a.w[q], red) : 0.0;
-        if (threadIdx.x == 0) out[C_WXX + q] = wq;
-    }
+    pair_partials<VIR>(partial + (size_t)blockIdx.x * NPART, a, fused, e_wall, e_fene, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -711,8 +747,9 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
     }
     if (tile == 0 && threadIdx.x == 0) *rebuild = 0;
     if (tile == P.block_base && threadIdx.x == 0) B.chan_rb[s] = 0;
-    // energy and pair count only on the step whose force is stored (k_pair64)
-    const bool calc_e = !fused || __builtin_amdgcn_readfirstlane(*B.store_frc) != 0;
+    // energy and pair count only on a fused call's last step (k_pair64)
+    const bool calc_e = fused == 0 || *B.t_idx >= *B.t_end;
+    if (fused != 0 && tile == 0 && threadIdx.x == 0) *B.store_frc = calc_e ? 1 : 0;
     __syncthreads();
     const int k = tile * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
@@ -720,6 +757,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
     a.fx = a.fy = a.fz = a.e = 0.0;
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
     a.np = 0;
+    double e_wall = 0.0, e_fene = 0.0;
     if (active) {
         const float4 pf = posf[k];
         const int ti = fwtype(pf.w), oi = fworig(pf.w);
@@ -829,17 +867,11 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
         }
         double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, X.t_idx, oi - P.part_base, ti, CUR(B.vel)[k], f);
+        if (fused == 2) bonded_epilogue(P, B, cur_off, pos[k], ti, f, e_wall, e_fene);
         frc[k] = f;
     }
-    double *out = partial + (size_t)tile * NPART;
     if (!calc_e) return;
-    double es = block_sum(a.e, red);
-    double nps = block_sum((double)a.np, red);
-    if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }
-    for (int q = 0; q < 6; ++q) {       // zero when the virial is off
-        double wq = VIR ? block_sum(a.w[q], red) : 0.0;
-        if (threadIdx.x == 0) out[C_WXX + q] = wq;
-    }
+    pair_partials<VIR>(partial + (size_t)tile * NPART, a, fused, e_wall, e_fene, red);
 }
 
 // Six waves per SIMD: caps the kernel at 80 VGPRs (91 without the hint, five

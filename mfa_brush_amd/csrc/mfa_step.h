@@ -77,29 +77,6 @@ __global__ __launch_bounds__(BLK) void k_velf_sync(const SysDev *__restrict__ sy
     CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
 }
 
-// ---------------------------------------------------------------------------
-// Bonded terms
-// ---------------------------------------------------------------------------
-struct BondOut { double fx, fy, fz, pot; };
-
-// FENE bond (a, b=a+1), intra_molec.f90:21-58: returns force_loc (the force
-// the reference adds to b and subtracts from a) and -0.5 ln(1-q).
-__device__ __forceinline__ BondOut fene(const SysDev &P, const double4 &ra, const double4 &rb, int *stretched) {
-#pragma clang fp contract(off)
-    BondOut o;
-    double dx = mic(ra.x - rb.x, P.L[0], P.invL[0]);
-    double dy = mic(ra.y - rb.y, P.L[1], P.invL[1]);
-    double dz = ra.z - rb.z;
-    double r_2 = r2_exact(dx, dy, dz);
-    double q = r_2 * P.inv_r_chain_2;
-    const double clamp = (double)0.98f;
-    if (q > clamp) { *stretched = 1; q = clamp; }
-    o.pot = -0.5 * log(1.0 - q);
-    double rd = P.k_chain / (1.0 - q) * P.inv_r_chain_2;
-    o.fx = rd * dx; o.fy = rd * dy; o.fz = rd * dz;
-    return o;
-}
-
 struct BendOut { double fp[3], fn[3], v; };
 
 // Harmonic-angle bending centred on c, bending.f90:30-109 (including the
@@ -160,14 +137,11 @@ __device__ __forceinline__ BendOut bend_angle(const SysDev &P, const double4 &rm
 // M_NOBEND: a fused step of channels that all have bending and melt bending
 // off; the bending code is compiled out (its registers cap the occupancy of
 // the fused kernel), its energy partials are still written (zero).
-enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64, M_NOBEND = 128 };
-
-__device__ __forceinline__ double4 bead_at(const Bufs &B, size_t cur_off, int gorig) {
-    if (!guard(B, in_range(gorig, B.n_beads), G_ORIG, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
-    const int k = B.where[gorig];
-    if (!guard(B, in_range(k, B.n_slots), G_WHERE, gorig)) return make_double4(0.0, 0.0, 1.0, 0.0);
-    return CUR(B.pos)[k];
-}
+// M_BENDF: the bending of the channels whose flags ask for it, added to the
+// stored force (the fused step with the fused integrator, between the pair
+// kernel and k_integrate).
+enum Mode { M_WALL = 1, M_FENE = 2, M_BEND = 4, M_BENDM = 8, M_CONST = 16, M_VV2 = 32, M_FUSED = 64, M_NOBEND = 128,
+            M_BENDF = 256 };
 
 // Bending contributions to the bead at chain position m (0-based) of a chain
 // of nm beads starting at global index g0, in the reference's loop order
@@ -212,7 +186,9 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     const int k = blockIdx.x * BLK + threadIdx.x;
     const bool active = (k - P.slot_base) < P.n_part;
     constexpr bool FUSED = (MODE & M_FUSED) != 0;
+    // M_BENDF always stores (k_integrate reads the force), sums only on the last step
     const bool store_f = !FUSED || *B.store_frc != 0;
+    const bool sums = (MODE & M_BENDF) ? *B.store_frc != 0 : store_f;
     double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
     if (active) {
         // no contraction: the kernel's variants (with and without the
@@ -232,34 +208,9 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         const int ol = brush ? o : o - P.pd;
         const int m = nm > 0 ? ol % nm : 0;
         const int g0 = g - m;
-        if (MODE & (M_WALL | M_FUSED)) {
-            // fluid_wall.f90:93-121; energy with the global a_w (Q9)
-            const double sw = P.sigma_wall[(ti - 1) & 3], aw = P.a_wall[(ti - 1) & 3];
-            double inv_z = 1.0 / ri.z;
-            double rd = sw * inv_z;
-            e_wall += fabs(aw) * ipow(rd, 9) - P.a_w * ipow(rd, 3);
-            F.z += 9.0 * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
-            F.z -= 3.0 * aw * ipow(sw, 3) * ipow(inv_z, 4);
-            inv_z = 1.0 / (P.L[2] - ri.z);
-            rd = sw * inv_z;
-            e_wall += fabs(aw) * ipow(rd, 9) - P.a_w * ipow(rd, 3);
-            F.z -= 9.0 * fabs(aw) * ipow(sw, 9) * ipow(inv_z, 10);
-            F.z += 3.0 * aw * ipow(sw, 3) * ipow(inv_z, 4);
-        }
-        if (MODE & (M_FENE | M_FUSED)) {
-            int str = 0;
-            if (m > 0) {
-                BondOut b = fene(P, bead_at(B, cur_off, g - 1), ri, &str);
-                F.x += b.fx; F.y += b.fy; F.z += b.fz;
-            }
-            if (m < nm - 1) {
-                BondOut b = fene(P, ri, bead_at(B, cur_off, g + 1), &str);
-                F.x -= b.fx; F.y -= b.fy; F.z -= b.fz;
-                e_fene += b.pot;
-            }
-            if (str) atomicAdd(&B.flags[2], 1);
-        }
-        constexpr bool FBEND = FUSED && !(MODE & M_NOBEND);
+        if (MODE & (M_WALL | M_FUSED)) wall_bead(P, ri, ti, F, e_wall);
+        if (MODE & (M_FENE | M_FUSED)) fene_bead(P, B, cur_off, ri, g, m, nm, F, e_fene);
+        constexpr bool FBEND = (FUSED && !(MODE & M_NOBEND)) || (MODE & M_BENDF) != 0;
         if ((MODE & M_BEND) || (FBEND && P.bending)) {
             if (brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bend);
         }
@@ -312,11 +263,11 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     // the fused step sums its energies only on the step whose force it
     // stores (the last of an md_steps call), the only one a caller can read
     double *out = partial + (size_t)blockIdx.x * NPART;
-    if (store_f) {
+    if (sums) {
         if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
         if (MODE & (M_FENE | M_FUSED)) { double t = block_sum(e_fene, red); if (threadIdx.x == 0) out[C_VINTRA] = t; }
-        if ((MODE & M_BEND) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
-        if ((MODE & M_BENDM) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
+        if ((MODE & (M_BEND | M_BENDF)) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
+        if ((MODE & (M_BENDM | M_BENDF)) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
         if (MODE & (M_VV2 | M_FUSED)) {
             double t = block_sum(ke, red);
             if (threadIdx.x == 0) out[C_KE] = t;
@@ -442,6 +393,112 @@ __global__ __launch_bounds__(BLK) void k_dpd_vv_apply(const SysDev *__restrict__
         f.x = f.x + d.x; f.y = f.y + d.y; f.z = f.z + d.z;
         CUR(B.frc)[k] = f;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused integrator of the fused step: the second half kick of step n
+// (verlet_velocities.f90:18-153, after constant_force(1)) and, unless step n
+// is the last of the md_steps call, the first half of step n + 1
+// (verlet_positions.f90:150-209 with the skin test of check_skin.f90:11-24).
+// The force of step n -- pair, walls, bonds (and bending) -- is complete in
+// frc when it runs; every bead touches only its own state, so positions are
+// updated in place.  On the call's last step it stores the force and the
+// acceleration and sums the kinetic energy (the pair kernel published that
+// decision in store_frc); otherwise the acceleration lives only in registers
+// between the two halves.
+// ---------------------------------------------------------------------------
+template <bool MIXED>
+__global__ __launch_bounds__(BLK) void k_integrate(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys,
+                                                  Bufs B, double *__restrict__ partial, double *__restrict__ f_heads) {
+    __shared__ double red[BLK];
+    const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
+    MFA_CUR(B, s);
+    const SysDev &P = sys[s];
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    const bool active = (k - P.slot_base) < P.n_part;
+    const bool last = *B.store_frc != 0;
+    if (!last && threadIdx.x == 0 && P.skin_mode == 0) { *B.rebuild = 1; B.chan_rb[s] = 1; }
+    double ke = 0.0;
+    if (active) {
+#pragma clang fp contract(off)
+        double4 r = CUR(B.pos)[k];
+        const int ti = wtype(r.w);
+        const int o = worig(r.w) - P.part_base;
+        const bool brush = o < P.pd;
+        const int ol = brush ? o : o - P.pd;
+        const bool head = brush && P.n_mon > 0 && ol % P.n_mon == 0;
+        double4 f4 = CUR(B.frc)[k];
+        double3 F = make_double3(f4.x, f4.y, f4.z);
+        if (P.poiseuille && ti != 1) {             // constant_force(1), md_main.f90:170
+            const double ms = P.mass_type[(ti - 1) & 3];
+            F.x += ms * P.const_force[0];
+            F.y += ms * P.const_force[1];
+            F.z += ms * P.const_force[2];
+        }
+        if (head && guard(B, in_range(P.head_base + ol / P.n_mon, B.n_heads), G_HEAD, k)) {
+            double *fh = f_heads + 3 * (size_t)(P.head_base + ol / P.n_mon);
+            fh[0] = F.x; fh[1] = F.y; fh[2] = F.z;
+        }
+        double4 v = CUR(B.vel)[k];
+        double4 a;
+        const double im = P.inv_mass_type[(ti - 1) & 3];
+        const double hd = 0.5 * P.dt;
+        a.x = F.x * im; a.y = F.y * im; a.z = F.z * im; a.w = 0.0;
+        v.x = v.x + hd * a.x; v.y = v.y + hd * a.y; v.z = v.z + hd * a.z;
+        if (head) {
+            if (P.poiseuille) { v.x = 0.0; v.y = 0.0; }
+            else {
+                const double sg = (ol / P.n_mon) < P.n_chain / 2 ? 1.0 : -1.0;
+                v.x = sg * P.head_vel[0]; v.y = sg * P.head_vel[1];
+            }
+            v.z = 0.0;
+            a.x = a.y = a.z = 0.0;
+            F.x = F.y = F.z = 0.0;
+        }
+        if (last) {
+            ke = P.mass_type[(ti - 1) & 3] * (v.x * v.x + v.y * v.y + v.z * v.z);
+            CUR(B.vel)[k] = v;
+            CUR(B.acc)[k] = a;
+            f4.x = F.x; f4.y = F.y; f4.z = F.z;
+            CUR(B.frc)[k] = f4;
+        } else {
+            // the next step's first half (k_vv1)
+            r.x = r.x + P.dt * v.x + (0.5 * P.dt_2) * a.x;
+            r.y = r.y + P.dt * v.y + (0.5 * P.dt_2) * a.y;
+            r.z = r.z + P.dt * v.z + (0.5 * P.dt_2) * a.z;
+            if (r.x > P.L[0]) r.x = r.x - P.L[0]; else if (r.x < 0.0) r.x = r.x + P.L[0];
+            if (r.y > P.L[1]) r.y = r.y - P.L[1]; else if (r.y < 0.0) r.y = r.y + P.L[1];
+            v.x = v.x + hd * a.x; v.y = v.y + hd * a.y; v.z = v.z + hd * a.z;
+            CUR(B.pos)[k] = r;
+            CUR(B.vel)[k] = v;
+            if (!isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z)) atomicAdd(&B.flags[0], 1);
+            if (r.z < 0.0 || r.z > P.L[2]) atomicAdd(&B.flags[1], 1);
+            if (P.skin_mode) {
+                const double4 b = CUR(B.posb)[k];
+                double dx = r.x - b.x, dy = r.y - b.y, dz = r.z - b.z;
+                dx -= P.L[0] * rint(dx * P.invL[0]);
+                dy -= P.L[1] * rint(dy * P.invL[1]);
+                if (!(dx * dx + dy * dy + dz * dz <= P.skin_half_2)) { atomicOr(B.rebuild, 1); atomicOr(&B.chan_rb[s], 1); }
+            }
+            if (MIXED) {
+                const int c = CUR(B.cell_of)[k] - P.cell_base;
+                if (guard(B, in_range(c, P.n_cells_tot), G_CELL, k)) {
+                    const int nx = P.nc[0], ny = P.nc[1];
+                    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+                    double rx = r.x - cx * P.l_cell[0], ry = r.y - cy * P.l_cell[1], rz = r.z - cz * P.l_cell[2];
+                    if (rx > 0.5 * P.L[0]) rx -= P.L[0]; else if (rx < -0.5 * P.L[0]) rx += P.L[0];
+                    if (ry > 0.5 * P.L[1]) ry -= P.L[1]; else if (ry < -0.5 * P.L[1]) ry += P.L[1];
+                    CUR(B.posf)[k] = make_float4((float)rx, (float)ry, (float)rz, mkwf(worig(r.w), wtype(r.w)));
+                    CUR(B.velf)[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+                }
+            }
+        }
+    }
+    if (last) {
+        double t = block_sum(ke, red);
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * NPART + C_KE] = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.t_idx += 1;
 }
 
 // Time index of the last of the n steps an md_steps call enqueues (the
