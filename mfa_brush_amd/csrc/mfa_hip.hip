@@ -330,7 +330,7 @@ static int alloc_list(mfa_ctx *c) {
     }
     if (cap == 0) return 0;
     cap = (cap + 7) / 8 * 8;      // rows of whole 16-byte groups (the pair kernels load 4 entries at once)
-    if (c->S_tot >= (1 << (31 - 7))) {
+    if (c->S_tot >= (1 << (32 - NL_SLOT_SHIFT))) {      // entries read back as unsigned
         if (c->nl_request > 0) return fail(MFA_E_ARG, "neighbour list: too many slots for the list encoding");
         return 0;
     }

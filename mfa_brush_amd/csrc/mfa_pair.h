@@ -45,10 +45,15 @@ struct PairAcc {
     int np;
 };
 
-// list entry: partner slot << NL_CODE_BITS | cell-offset code, the code being
-// (dx+S) + (2S+1)((dy+S) + (2S+1)(dz+S)) for the partner's cell relative to
-// the bead's cell
-constexpr int NL_CODE_BITS = 7;
+// list entry: partner slot << NL_SLOT_SHIFT | (partner type - 1) << 7 |
+// cell-offset code, the code being (dx+S) + (2S+1)((dy+S) + (2S+1)(dz+S)) for
+// the partner's cell relative to the bead's cell (slots below 2^23)
+constexpr int NL_SLOT_SHIFT = 9;
+constexpr int NL_TYPE_SHIFT = 7;
+constexpr int NL_CODE_MASK = 0x7f;
+__device__ __forceinline__ int nl_entry(int slot, int type, int code) {
+    return (int)(((unsigned)slot << NL_SLOT_SHIFT) | ((unsigned)((type - 1) & 3) << NL_TYPE_SHIFT) | (unsigned)code);
+}
 constexpr int NL_UNROLL = 4;      // partner loads issued together (one 16-byte list load)
 constexpr int NL_STAGE = 512;     // partner beads a list-build block stages in LDS per row of cells
 constexpr int DPD_Q = 8;          // LDS queue of deferred DPD pairs per lane
@@ -61,7 +66,7 @@ __device__ __forceinline__ size_t nl_index(int k, int q, size_t stride) {
 }
 // Slot i of a per-slot array through an unsigned 32-bit byte offset, so the
 // load takes the uniform base in scalar registers and one 32-bit lane offset
-// (no 64-bit address add per gather).  Lists hold slots below 2^24 (the host
+// (no 64-bit address add per gather).  Lists hold slots below 2^23 (the host
 // refuses more), so i * 32 stays below 2^32.
 template <typename T>
 __device__ __forceinline__ T ld_slot(const T *__restrict__ base, unsigned i) {
@@ -346,7 +351,9 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
                         if constexpr (decltype(tail)::value) ok = ok && m < rg.y;
                         if constexpr (decltype(self)::value) ok = ok && m != k;
                         if (ok) {
-                            b0 = b1; b1 = b2; b2 = b3; b3 = (m << NL_CODE_BITS) | ecode;
+                            int tj;
+                            if constexpr (MIXED) tj = fwtype(pj[u].w); else tj = wtype(pj[u].w);
+                            b0 = b1; b1 = b2; b2 = b3; b3 = nl_entry(m, tj, ecode);
                             if ((n & 3) == 3) {
                                 if (n < cap) *gp = make_int4(b0, b1, b2, b3);
                                 gp += group_step;
@@ -495,8 +502,8 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
                 double4 pj[NL_UNROLL];
                 const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
                 // unsigned slots: one compare is the range check (as in k_pair32)
-                mm[0] = (unsigned)e4.x >> NL_CODE_BITS; mm[1] = (unsigned)e4.y >> NL_CODE_BITS;
-                mm[2] = (unsigned)e4.z >> NL_CODE_BITS; mm[3] = (unsigned)e4.w >> NL_CODE_BITS;
+                mm[0] = (unsigned)e4.x >> NL_SLOT_SHIFT; mm[1] = (unsigned)e4.y >> NL_SLOT_SHIFT;
+                mm[2] = (unsigned)e4.z >> NL_SLOT_SHIFT; mm[3] = (unsigned)e4.w >> NL_SLOT_SHIFT;
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) pj[u] = ld_slot(pos, min(mm[u], nsl - 1u));
 #pragma unroll
@@ -616,7 +623,7 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const 
     bool lj = r2 < T[F_LO * 32 + ix];
     if (!lj && r2 < T[F_HI * 32 + ix]) {
         // borderline: decide with the FP64 reference arithmetic
-        const int tt = ((ti - 1) & 3) + 4 * ((fwtype(pj.w) - 1) & 3);
+        const int tt = ((ti - 1) & 3) + 4 * (((ix & 7) - 1) & 3);
         const double4 pa = pos[k], pb = pos[m];
         double ex = mic(pa.x - pb.x, P.L[0], P.invL[0]);
         double ey = mic(pa.y - pb.y, P.L[1], P.invL[1]);
@@ -769,11 +776,11 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
         unsigned short *const myc = dq_c + threadIdx.x;
         const int dcap = B.dpd_queue;
         int dq = 0;
-        // LJ part of candidate m (position pj, cell-offset code); a DPD pair
-        // is queued, or evaluated in place when the queue is full
-        auto visit = [&](int m, int code, float4 pj) {
+        // LJ part of candidate m (position pj, cell-offset code, table index
+        // ix); a DPD pair is queued, or evaluated in place when the queue is
+        // full (the partner's identity word reloaded with its position)
+        auto visit = [&](int m, int code, float4 pj, int ix) {
             const float4 o = off[code];
-            const int ix = trow8 | (int)(__float_as_uint(pj.w) & 7u);
             const int f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, calc_e, a);
             if (DPD && f) {
                 if (dq < dcap) {
@@ -781,7 +788,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
                     myc[dq * BLK] = (unsigned short)(code | (f == 1 ? 0x80 : 0));
                     ++dq;
                 } else {
-                    pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, pj, velf[m], f == 1, ix, a);
+                    pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, posf[m], velf[m], f == 1, ix, a);
                 }
             }
         };
@@ -791,21 +798,25 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
             // checked cell ranges); the clamp only keeps a gather in bounds
             const unsigned last = (unsigned)B.n_slots - 1u;
             // one 16-byte group of four entries, their four gathers in
-            // flight together; TAIL: the list's last, partial group
+            // flight together; TAIL: the list's last, partial group.  The
+            // partner's type comes with the entry, so only its three cell-
+            // frame coordinates are gathered (12 of the 16 bytes).
+            const int trow8p1 = trow8 + 1;
             auto group = [&](int q0, auto tail) {
                 const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
                 const int ent[NL_UNROLL] = {e4.x, e4.y, e4.z, e4.w};
                 unsigned mm[NL_UNROLL];
-                float4 pj[NL_UNROLL];
+                float3 pj[NL_UNROLL];
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    mm[u] = min((unsigned)ent[u] >> NL_CODE_BITS, last);
-                    pj[u] = ld_slot(posf, mm[u]);
+                    mm[u] = min((unsigned)ent[u] >> NL_SLOT_SHIFT, last);
+                    pj[u] = *(const float3 *)((const char *)posf + mm[u] * (unsigned)sizeof(float4));
                 }
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
                     if (!decltype(tail)::value || q0 + u < nn)
-                        visit((int)mm[u], ent[u] & ((1 << NL_CODE_BITS) - 1), pj[u]);
+                        visit((int)mm[u], ent[u] & NL_CODE_MASK, make_float4(pj[u].x, pj[u].y, pj[u].z, 0.f),
+                              trow8p1 + ((ent[u] >> NL_TYPE_SHIFT) & 3));
                 }
             };
             int q0 = 0;
@@ -833,7 +844,8 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
                         const int code = nl_code(dx, dy, dz, S);
                         for (int m = rg.x; m < rg.y; ++m) {
                             if (m == k) continue;
-                            visit(m, code, posf[m]);
+                            const float4 pj = posf[m];
+                            visit(m, code, pj, trow8 | (int)(__float_as_uint(pj.w) & 7u));
                         }
                     }
                 }

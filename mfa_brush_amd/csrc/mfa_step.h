@@ -339,7 +339,7 @@ __global__ __launch_bounds__(BLK) void k_dpd_vv(const SysDev *__restrict__ sys, 
     const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
     if (nn >= 0 && nn <= B.nl_cap) {
         for (int q = 0; q < nn; ++q) {
-            const unsigned m = (unsigned)B.nl[nl_index(k, q, (size_t)B.n_slots)] >> NL_CODE_BITS;
+            const unsigned m = (unsigned)B.nl[nl_index(k, q, (size_t)B.n_slots)] >> NL_SLOT_SHIFT;
             if (guard(B, m < (unsigned)B.n_slots, G_SLOT, k)) visit((int)m);
         }
     } else {
