@@ -264,6 +264,20 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
     return out
 
 
+def main_tte(argv):
+    """--tte report.json [n_train equil avg n_blocks side sigma_g n_mon eps_bs]:
+    one bounded_tte run, written as JSON."""
+    out_path = argv[0]
+    vals = [float(x) for x in argv[1:]]
+    d = [24, 20000, 10000, 100, 20.0, 0.25, 16, 1.0]
+    vals = vals + d[len(vals):]
+    held = Spec(vals[5], int(vals[6]), vals[7], side=vals[4], seed=907)
+    r = bounded_tte(n_train=int(vals[0]), equil=int(vals[1]), avg=int(vals[2]), n_blocks=int(vals[3]), held=held)
+    print(json.dumps(r), flush=True)
+    with open(out_path, "w") as f:
+        json.dump(r, f, indent=1)
+
+
 def main(argv=None):
     """Train on batched runs, then compare plain and warm starts on held-out
     channels.  Arguments: report path, training channels, equilibration and
@@ -273,6 +287,8 @@ def main(argv=None):
     comma-separated held-out indices (0-2, default all).  Writes a JSON
     report."""
     argv = sys.argv[1:] if argv is None else argv
+    if argv and argv[0] == "--tte":
+        return main_tte(argv[1:])
     out_path = argv[0] if argv else "warmstart_report.json"
     n_train = int(argv[1]) if len(argv) > 1 else 48
     equil, avg = (int(argv[2]), int(argv[3])) if len(argv) > 3 else (40000, 20000)
