@@ -800,10 +800,15 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
             // one 16-byte group of four entries, their four gathers in
             // flight together; TAIL: the list's last, partial group.  The
             // partner's type comes with the entry, so only its three cell-
-            // frame coordinates are gathered (12 of the 16 bytes).
+            // frame coordinates are gathered (12 of the 16 bytes).  The next
+            // group's entries are loaded after this group's gathers are
+            // issued, so their memory latency overlaps this group's pair
+            // arithmetic (the group past a bead's last one, clamped to the
+            // row's last group, is loaded and not used).
             const int trow8p1 = trow8 + 1;
+            const int q_last = B.nl_cap - NL_UNROLL;
+            int4 e4 = *(const int4 *)(B.nl + nl_index(k, 0, (size_t)B.n_slots));
             auto group = [&](int q0, auto tail) {
-                const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
                 const int ent[NL_UNROLL] = {e4.x, e4.y, e4.z, e4.w};
                 unsigned mm[NL_UNROLL];
                 float3 pj[NL_UNROLL];
@@ -812,6 +817,8 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
                     mm[u] = min((unsigned)ent[u] >> NL_SLOT_SHIFT, last);
                     pj[u] = *(const float3 *)((const char *)posf + mm[u] * (unsigned)sizeof(float4));
                 }
+                if (!decltype(tail)::value)
+                    e4 = *(const int4 *)(B.nl + nl_index(k, min(q0 + NL_UNROLL, q_last), (size_t)B.n_slots));
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
                     if (!decltype(tail)::value || q0 + u < nn)
