@@ -8,17 +8,18 @@ prediction for a new channel seeds the initial configuration
 (configs.channel_general: monomer heights from the predicted brush density,
 solvent sites weighted by the predicted solvent density).  The plain lattice
 start (straight chains) remains the fallback, and plain MD remains the judge:
-time to equilibrium is measured on the simulated profiles of both starts.
+time to equilibrium is measured on the simulated profiles of each start.
 
 Time to equilibrium: the first block of `block` steps after which the
 block-averaged brush and solvent profiles (n_bins bins over [0, Lz], mirrored
 about the mid-plane) stay within `tol` (L1 distance relative to the L1 norm of
 the reference) for `consecutive` blocks.  The reference is the equilibrium
-estimate from the second half of both runs together (plain and warm start,
-equal weight); the L1 distance between the two runs' second-half means is
-reported with it, as the check that both reached the same state.  No
-counterpart in the reference code (its starts come from gen_brush.f90 /
-conf_default.f90).
+estimate from the second halves of the runs together (equal weight); the
+largest L1 distance between two runs' second-half means is reported with it,
+as the check that the runs reached the same state.  Three starts are
+compared: the plain lattice start, the reference's own generator start
+(gen_brush.f90 mode 5 and gen_droplet.f90 case 1, restated in
+generators.py) and the warm start.  No counterpart in the reference code.
 """
 from __future__ import annotations
 
@@ -31,7 +32,7 @@ from dataclasses import dataclass, asdict
 
 import numpy as np
 
-from . import configs
+from . import configs, generators
 from .sim import Simulation
 from .surrogate import Mlp
 
@@ -91,6 +92,18 @@ def build(spec: Spec, mlp: Mlp | None = None, skin: float = 0.3):
 
     return _soft(configs.channel_general(spec.n_side(), spec.a_graft(), spec.n_mon, LZ, brush_z=brush_z,
                                          solvent_weight=solvent_weight, **kw))
+
+
+def build_generated(spec: Spec, skin: float = 0.3):
+    """The reference's own start for the channel (generators.conf_default):
+    the ordered two-wall brush of gen_brush mode 5 (heads on the square
+    lattice of spacing a_graft, chains as random walks), solvent beads placed
+    uniformly by gen_droplet case 1 without an overlap check, and
+    Maxwell-Boltzmann velocities; same bead counts, types and force switch-on
+    as the plain start."""
+    p, _, _, _ = build(spec, None, skin)
+    a_type, r0, v = generators.conf_default(p, brush_mode=5, seed=spec.seed)
+    return p, a_type, r0, v
 
 
 def _bins():
@@ -164,34 +177,11 @@ def train(specs, profiles, steps: int = 3000, seed: int = 5):
     return mlp, hist
 
 
-def block_profiles(spec: Spec, mlp, n_blocks: int, block: int = 1000, every: int = 50, skin: float = 0.3):
-    """Block-averaged (brush, solvent) densities of one channel run."""
-    p, t, r0, v = build(spec, mlp, skin)
-    print("  %s start built: %d beads" % ("warm" if mlp is not None else "plain", p.n_part), flush=True)
-    out = []
-    t0 = time.perf_counter()
-    with Simulation(p) as sim:
-        sim.set_state(r0, v, t)
-        for b in range(n_blocks):
-            sim.profiles_reset(N_BINS)
-            for q in range(block // every):
-                sim.md_steps(every, graph=True)
-                sim.profiles_accumulate()
-            n_obs, dens, _, _ = sim.profiles()
-            out.append(densities(p, dens, n_obs))
-            if (b + 1) % 25 == 0:
-                print("  block %d/%d" % (b + 1, n_blocks), flush=True)
-        sim.synchronize()
-        fl = sim.flags()
-    return np.array(out), p.n_part, time.perf_counter() - t0, fl
-
-
-def block_profiles_both(spec: Spec, mlp, n_blocks: int, block: int = 1000, every: int = 50, skin: float = 0.3):
-    """Block-averaged (brush, solvent) densities of the plain start and the
-    warm start of one channel, run side by side in one batched context (each
-    channel's trajectory is the one it has alone)."""
-    starts = [build(spec, None, skin), build(spec, mlp, skin)]
-    out = [[], []]
+def block_profiles_starts(starts, n_blocks: int, block: int = 1000, every: int = 50):
+    """Block-averaged (brush, solvent) densities of several starts of one
+    channel (each a (p, a_type, r0, v) tuple), run side by side in one batched
+    context (each channel's trajectory is the one it has alone)."""
+    out = [[] for _ in starts]
     t0 = time.perf_counter()
     with Simulation([b[0] for b in starts]) as sim:
         for i, (p, t, r0, v) in enumerate(starts):
@@ -206,7 +196,7 @@ def block_profiles_both(spec: Spec, mlp, n_blocks: int, block: int = 1000, every
                 out[i].append(densities(p, dens, n_obs))
         sim.synchronize()
         fl = sim.flags()
-    return np.array(out[0]), np.array(out[1]), starts[0][0].n_part, time.perf_counter() - t0, fl
+    return [np.array(o) for o in out], starts[0][0].n_part, time.perf_counter() - t0, fl
 
 
 def l1_error(prof, ref):
@@ -223,16 +213,21 @@ def time_to_equilibrium(blocks, ref, tol: float = 0.02, consecutive: int = 5):
     return None, err
 
 
+START_NAMES = ("plain", "generated", "warm")
+
+
 def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 100, block: int = 1000,
                 mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21):
     """The whole warm-start method at a bounded size (tens of seconds on one
     GPU): n_train small channels (12 sigma boxes, N in {8, 16}) run batched
     for equil + avg steps to give training profiles, the surrogate trains on
     them, and a held-out channel runs n_blocks blocks of `block` steps from
-    the plain and from the warm start.  Returns the first-block L1 errors,
-    the surrogate's own L1 error and the blocks to the 10 / 5 / 2 % bands
-    (5 consecutive blocks) of both starts, against the mean of the second
-    halves of both runs."""
+    three starts side by side: the plain lattice start, the reference's
+    generator start (build_generated) and the warm start.  Returns the
+    first-block L1 errors, the surrogate's own L1 error and the blocks to the
+    10 / 5 / 2 % bands (5 consecutive blocks) of each start, against the mean
+    of the second halves of all three runs, with the warm start's speedup
+    over each of the other two."""
     t0 = time.perf_counter()
     rng = np.random.default_rng(seed)
     specs = [Spec(sigma_g=float(rng.uniform(0.05, 0.5)), n_mon=int(rng.choice([8, 16])),
@@ -241,26 +236,27 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
     t_sim = time.perf_counter() - t0
     mlp, hist = train(specs, profs, steps=mlp_steps)
     held = held or Spec(0.25, 16, 1.0, side=20.0, seed=907)
-    plain, warm, n_part, secs, fl = block_profiles_both(held, mlp, n_blocks, block=block)
-    late_p, late_w = plain[n_blocks // 2:].mean(0), warm[n_blocks // 2:].mean(0)
-    ref = 0.5 * (late_p + late_w)
+    starts = [build(held, None), build_generated(held), build(held, mlp)]
+    runs, n_part, secs, fl = block_profiles_starts(starts, n_blocks, block=block)
+    late = [r[n_blocks // 2:].mean(0) for r in runs]
+    ref = sum(late) / len(late)
     zc, rb, rs = predict(mlp, held)
-    err_p = [l1_error(b, ref) for b in plain]
-    err_w = [l1_error(b, ref) for b in warm]
     out = {"held_out": asdict(held), "n_part": n_part, "block_steps": block, "blocks": n_blocks,
-           "l1_every_10th_block_plain": [round(e, 4) for e in err_p[::10]],
-           "l1_every_10th_block_warm": [round(e, 4) for e in err_w[::10]],
            "train_channels": n_train, "train_beads": n_beads, "train_steps": equil + avg,
            "mlp_loss_first_last": [float(hist[:20].mean()), float(hist[-20:].mean())],
-           "first_block_l1_plain": l1_error(plain[0], ref), "first_block_l1_warm": l1_error(warm[0], ref),
-           "surrogate_l1": l1_error(np.stack([rb, rs]), ref), "late_gap_l1": l1_error(late_w, late_p),
+           "surrogate_l1": l1_error(np.stack([rb, rs]), ref),
+           "late_gap_l1": max(l1_error(a, b) for a in late for b in late),
            "nonfinite": int(fl["nonfinite"]), "seconds": time.perf_counter() - t0,
            "train_sim_seconds": t_sim, "held_out_seconds": secs}
+    for name, r in zip(START_NAMES, runs):
+        out["first_block_l1_" + name] = l1_error(r[0], ref)
+        out["l1_every_10th_block_" + name] = [round(l1_error(b, ref), 4) for b in r[::10]]
     for tol in (0.10, 0.05, 0.02):
-        tp, _ = time_to_equilibrium(plain, ref, tol=tol)
-        tw, _ = time_to_equilibrium(warm, ref, tol=tol)
-        out["tte_blocks_%d%%" % round(100 * tol)] = {"plain": tp, "warm": tw,
-                                                      "speedup": (tp / tw) if (tp and tw) else None}
+        tte = {name: time_to_equilibrium(r, ref, tol=tol)[0] for name, r in zip(START_NAMES, runs)}
+        tw = tte["warm"]
+        for name in ("plain", "generated"):
+            tte["speedup_vs_" + name] = (tte[name] / tw) if (tte[name] and tw) else None
+        out["tte_blocks_%d%%" % round(100 * tol)] = tte
     return out
 
 
@@ -320,20 +316,22 @@ def main(argv=None):
             Spec(0.45, 32, 1.3, side=side, seed=903)]
     rep["held_out"] = []
     for s in [held[i] for i in which]:
-        plain, n_part, secs_p, flp = block_profiles(s, None, n_blocks)
-        warm, _, secs_w, flw = block_profiles(s, mlp, n_blocks)
-        late_p, late_w = plain[n_blocks // 2:].mean(0), warm[n_blocks // 2:].mean(0)
-        ref = 0.5 * (late_p + late_w)
-        tp, ep = time_to_equilibrium(plain, ref)
-        tw, ew = time_to_equilibrium(warm, ref)
+        starts = [build(s, None), build_generated(s), build(s, mlp)]
+        runs, n_part, secs, fl = block_profiles_starts(starts, n_blocks)
+        late = [r[n_blocks // 2:].mean(0) for r in runs]
+        ref = sum(late) / len(late)
         zc, rb, rs = predict(mlp, s)
-        pred_err = l1_error(np.stack([rb, rs]), ref)
         r = {"spec": asdict(s), "n_part": n_part, "blocks": n_blocks, "block_steps": 1000,
-             "tte_plain_blocks": tp, "tte_warm_blocks": tw, "speedup": (tp / tw) if (tp and tw) else None,
-             "surrogate_l1_vs_reference": pred_err, "late_gap_plain_warm_l1": l1_error(late_w, late_p),
-             "err_plain": ep, "err_warm": ew, "seconds": [secs_p, secs_w],
-             "flags": [flp, flw]}
-        print(json.dumps(r), flush=True)
+             "surrogate_l1_vs_reference": l1_error(np.stack([rb, rs]), ref),
+             "late_gap_l1": max(l1_error(a, b) for a in late for b in late), "seconds": secs, "flags": fl}
+        for tol in (0.10, 0.05, 0.02):
+            tte = {name: time_to_equilibrium(x, ref, tol=tol)[0] for name, x in zip(START_NAMES, runs)}
+            for name in ("plain", "generated"):
+                tte["speedup_vs_" + name] = (tte[name] / tte["warm"]) if (tte[name] and tte["warm"]) else None
+            r["tte_blocks_%d%%" % round(100 * tol)] = tte
+        for name, x in zip(START_NAMES, runs):
+            r["err_" + name] = [l1_error(b, ref) for b in x]
+        print(json.dumps({k: v for k, v in r.items() if not k.startswith("err_")}), flush=True)
         rep["held_out"].append(r)
     with open(out_path, "w") as f:
         json.dump(rep, f, indent=1)
