@@ -85,11 +85,16 @@ typedef struct mfa_energies {
     double v_bend;          /* bending.f90:100 */
     double v_bend_melt;     /* bending_melt.f90 */
     double t_fluid;         /* 0.5 sum m v^2, verlet_velocities.f90:132-153 */
-    double virial[6];       /* pair virial W = sum over interacting pairs of r_ij (x) F_ij, xx yy zz xy xz yz,
-                               r_ij the minimum-image separation and F_ij the whole pair force: LJ, DPD
-                               dissipative and DPD random parts (the reference's bulk-mode press_tensor,
-                               dpd_forces.f90:118-127, keeps only the dissipative DPD part; it has none in
-                               channel mode).  Zero while mfa_hip_set_virial is off */
+    double virial[6];       /* virial W = sum over interacting pairs and FENE bonds of r_ij (x) F_ij, xx yy zz
+                               xy xz yz, r_ij the minimum-image separation i - j and F_ij the force on i from
+                               j: for pairs the whole pair force, LJ, DPD dissipative and DPD random parts (the
+                               reference's bulk-mode press_tensor, dpd_forces.f90:118-127, keeps only the
+                               dissipative DPD part; it has none in channel mode); for bonds the FENE force
+                               (intra_molec.f90:61-66,115-120, whose brush loop adds the opposite sign).  The
+                               pair part is that of the last fluid_fluid (or fused step), the bond part that
+                               of the intra_molec after it (or of the fused step); walls and bending add
+                               none.  Zero while mfa_hip_set_virial is off */
+    double virial_bond[6];  /* the FENE bond part of virial */
 } mfa_energies;
 
 typedef struct mfa_flags {

@@ -44,7 +44,7 @@ class SystemParams(C.Structure):
 class Energies(C.Structure):
     _fields_ = [("v_fluid_fluid", C.c_double), ("v_fluid_wall", C.c_double), ("v_intra_molec", C.c_double),
                 ("v_bend", C.c_double), ("v_bend_melt", C.c_double), ("t_fluid", C.c_double),
-                ("virial", C.c_double * 6)]
+                ("virial", C.c_double * 6), ("virial_bond", C.c_double * 6)]
 
 
 class Flags(C.Structure):

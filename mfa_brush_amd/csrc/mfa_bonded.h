@@ -15,7 +15,7 @@ __device__ __forceinline__ double4 bead_at(const Bufs &B, size_t cur_off, int go
 // ---------------------------------------------------------------------------
 // Bonded terms
 // ---------------------------------------------------------------------------
-struct BondOut { double fx, fy, fz, pot; };
+struct BondOut { double fx, fy, fz, pot, dx, dy, dz; };
 
 // FENE bond (a, b=a+1), intra_molec.f90:21-58: returns force_loc (the force
 // the reference adds to b and subtracts from a) and -0.5 ln(1-q).
@@ -32,7 +32,18 @@ __device__ __forceinline__ BondOut fene(const SysDev &P, const double4 &ra, cons
     o.pot = -0.5 * log(1.0 - q);
     double rd = P.k_chain / (1.0 - q) * P.inv_r_chain_2;
     o.fx = rd * dx; o.fy = rd * dy; o.fz = rd * dz;
+    o.dx = dx; o.dy = dy; o.dz = dz;
     return o;
+}
+
+// Virial r_ab (x) F_ab of a FENE bond, added to w (xx yy zz xy xz yz): r_ab
+// the minimum-image separation a - b and F_ab = -force_loc the force on a.
+// Both beads of a bond add the same value (the bond seen from b negates both
+// vectors), so a sum over beads counts every bond twice.
+__device__ __forceinline__ void bond_virial(const BondOut &b, double *w) {
+#pragma clang fp contract(off)
+    w[0] -= b.fx * b.dx; w[1] -= b.fy * b.dy; w[2] -= b.fz * b.dz;
+    w[3] -= b.fx * b.dy; w[4] -= b.fx * b.dz; w[5] -= b.fy * b.dz;
 }
 
 // fluid_wall(2), fluid_wall.f90:93-121, on bead ri of type ti: the 9-3 force
@@ -56,18 +67,22 @@ __device__ __forceinline__ void wall_bead(const SysDev &P, const double4 &ri, in
 // intra_molec, intra_molec.f90:16-125, on the bead at chain position m of a
 // chain of nm beads (global index g): the bond to the previous bead then the
 // bond to the next one; the bond's energy goes to the bead that starts it.
+// VIR: each bond's virial added to wb (twice per bond over all beads).
+template <bool VIR = false>
 __device__ __forceinline__ void fene_bead(const SysDev &P, const Bufs &B, size_t cur_off, const double4 &ri, int g,
-                                          int m, int nm, double3 &F, double &e_fene) {
+                                          int m, int nm, double3 &F, double &e_fene, double *wb = nullptr) {
 #pragma clang fp contract(off)
     int str = 0;
     if (m > 0) {
         BondOut b = fene(P, bead_at(B, cur_off, g - 1), ri, &str);
         F.x += b.fx; F.y += b.fy; F.z += b.fz;
+        if (VIR) bond_virial(b, wb);
     }
     if (m < nm - 1) {
         BondOut b = fene(P, ri, bead_at(B, cur_off, g + 1), &str);
         F.x -= b.fx; F.y -= b.fy; F.z -= b.fz;
         e_fene += b.pot;
+        if (VIR) bond_virial(b, wb);
     }
     if (str) atomicAdd(&B.flags[2], 1);
 }

@@ -420,9 +420,11 @@ template <int MODE>
 static void enqueue_bonded(mfa_ctx *c) {
     Timer t(c, K_BONDED);
     if (c->prec == MFA_PREC_MIXED)
-        LAUNCH((k_bonded<MODE, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        LAUNCH((k_bonded<MODE, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads,
+                   (int)c->virial);
     else
-        LAUNCH((k_bonded<MODE, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+        LAUNCH((k_bonded<MODE, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads,
+                   (int)c->virial);
 }
 
 // DPD-VV re-evaluation after the second half kick (new_dpd_fd.f90); all:
@@ -460,9 +462,11 @@ static void enqueue_step(mfa_ctx *c) {
         Timer t(c, K_BONDED);
         if (any_bending(c)) {
             if (c->prec == MFA_PREC_MIXED)
-                LAUNCH((k_bonded<M_BENDF, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+                LAUNCH((k_bonded<M_BENDF, true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads,
+                   (int)c->virial);
             else
-                LAUNCH((k_bonded<M_BENDF, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
+                LAUNCH((k_bonded<M_BENDF, false>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads,
+                   (int)c->virial);
         }
         if (c->prec == MFA_PREC_MIXED)
             LAUNCH((k_integrate<true>), c->n_blocks, c->st, c->d_sys, c->d_blk_sys, c->B, c->d_partial, c->d_fheads);
@@ -984,7 +988,10 @@ int mfa_hip_get_energies(mfa_ctx *c, int32_t isys, mfa_energies *out) {
     out->v_bend = e[C_VBEND];
     out->v_bend_melt = e[C_VBENDM];
     out->t_fluid = 0.5 * e[C_KE];
-    for (int q = 0; q < 6; ++q) out->virial[q] = 0.5 * e[C_WXX + q];
+    for (int q = 0; q < 6; ++q) {
+        out->virial_bond[q] = 0.5 * e[C_WBXX + q];
+        out->virial[q] = 0.5 * (e[C_WXX + q] + e[C_WBXX + q]);
+    }
     return 0;
 }
 

@@ -18,9 +18,12 @@
 namespace mfa {
 
 constexpr int BLK = 256;          // threads per block and slot-range alignment
-constexpr int NPART = 16;         // partial-sum columns per block
+constexpr int NPART = 24;         // partial-sum columns per block
+// energies, kinetic energy, pair virial (C_W*), pair count, FENE bond virial (C_WB*)
 enum Col { C_VFF = 0, C_VWALL, C_VINTRA, C_VBEND, C_VBENDM, C_KE,
-           C_WXX, C_WYY, C_WZZ, C_WXY, C_WXZ, C_WYZ, C_NPAIR, C_NCOL };
+           C_WXX, C_WYY, C_WZZ, C_WXY, C_WXZ, C_WYZ, C_NPAIR,
+           C_WBXX, C_WBYY, C_WBZZ, C_WBXY, C_WBXZ, C_WBYZ, C_NCOL };
+static_assert(C_NCOL <= NPART, "partial-sum columns");
 
 struct SysDev {
     double L[3], invL[3];

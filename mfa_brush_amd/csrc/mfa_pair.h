@@ -395,8 +395,9 @@ __global__ __launch_bounds__(BLK) void k_nlist(const SysDev *__restrict__ sys, c
 // walls and the FENE bonds of bead k (FP64 position ri, global index g, type
 // ti) added to its pair force f in md_main.f90's order (fluid_wall :132,
 // intra_molec :142), energies to e_wall and e_fene.
+template <bool VIR>
 __device__ __forceinline__ void bonded_epilogue(const SysDev &P, const Bufs &B, size_t cur_off, const double4 &ri,
-                                                int ti, double4 &f, double &e_wall, double &e_fene) {
+                                                int ti, double4 &f, double &e_wall, double &e_fene, double *wb) {
     const int g = worig(ri.w);
     const int o = g - P.part_base;
     const bool brush = o < P.pd;
@@ -405,22 +406,25 @@ __device__ __forceinline__ void bonded_epilogue(const SysDev &P, const Bufs &B, 
     const int m = nm > 0 ? ol % nm : 0;
     double3 F = make_double3(f.x, f.y, f.z);
     wall_bead(P, ri, ti, F, e_wall);
-    fene_bead(P, B, cur_off, ri, g, m, nm, F, e_fene);
+    fene_bead<VIR>(P, B, cur_off, ri, g, m, nm, F, e_fene, wb);
     f.x = F.x; f.y = F.y; f.z = F.z;
 }
 
 // Energy partials of a pair-kernel block: the pair energy and count, the
-// virial (zero when off) and, in mode 2, the walls and bonds the epilogue
-// added (bending columns zeroed; the bending kernel, if any, writes them).
+// pair virial (zero when off), the FENE bond virial (the epilogue's in mode 2,
+// else zero: the bonded kernel of the step writes it) and, in mode 2, the
+// walls and bonds the epilogue added (bending columns zeroed; the bending
+// kernel, if any, writes them).
 template <bool VIR>
 __device__ __forceinline__ void pair_partials(double *out, const PairAcc &a, int mode, double e_wall, double e_fene,
-                                              double *red) {
+                                              const double *wb, double *red) {
     double es = block_sum(a.e, red);
     double nps = block_sum((double)a.np, red);
     if (threadIdx.x == 0) { out[C_VFF] = es; out[C_NPAIR] = nps; }
     for (int q = 0; q < 6; ++q) {       // zero when the virial is off
         double wq = VIR ? block_sum(a.w[q], red) : 0.0;
-        if (threadIdx.x == 0) out[C_WXX + q] = wq;
+        double bq = (VIR && mode == 2) ? block_sum(wb[q], red) : 0.0;
+        if (threadIdx.x == 0) { out[C_WXX + q] = wq; out[C_WBXX + q] = bq; }
     }
     if (mode == 2) {
         double ew = block_sum(e_wall, red);
@@ -472,6 +476,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
     a.np = 0;
     double e_wall = 0.0, e_fene = 0.0;
+    double wb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
         const double4 pi = pos[k];
         const int ti = wtype(pi.w), oi = worig(pi.w);
@@ -562,11 +567,11 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
         }
         double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, (uint64_t)t_idx, oi - P.part_base, ti, vel[k], f);
-        if (fused == 2) bonded_epilogue(P, B, cur_off, pi, ti, f, e_wall, e_fene);
+        if (fused == 2) bonded_epilogue<VIR>(P, B, cur_off, pi, ti, f, e_wall, e_fene, wb);
         frc[k] = f;
     }
     if (!calc_e) return;
-    pair_partials<VIR>(partial + (size_t)blockIdx.x * NPART, a, fused, e_wall, e_fene, red);
+    pair_partials<VIR>(partial + (size_t)blockIdx.x * NPART, a, fused, e_wall, e_fene, wb, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -765,6 +770,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
     for (int q = 0; q < 6; ++q) a.w[q] = 0.0;
     a.np = 0;
     double e_wall = 0.0, e_fene = 0.0;
+    double wb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
         const float4 pf = posf[k];
         const int ti = fwtype(pf.w), oi = fworig(pf.w);
@@ -886,11 +892,11 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
         }
         double4 f = make_double4(a.fx, a.fy, a.fz, 0.0);
         if (P.thermostat == 2) lgv_bead(P, X.t_idx, oi - P.part_base, ti, CUR(B.vel)[k], f);
-        if (fused == 2) bonded_epilogue(P, B, cur_off, pos[k], ti, f, e_wall, e_fene);
+        if (fused == 2) bonded_epilogue<VIR>(P, B, cur_off, pos[k], ti, f, e_wall, e_fene, wb);
         frc[k] = f;
     }
     if (!calc_e) return;
-    pair_partials<VIR>(partial + (size_t)tile * NPART, a, fused, e_wall, e_fene, red);
+    pair_partials<VIR>(partial + (size_t)tile * NPART, a, fused, e_wall, e_fene, wb, red);
 }
 
 // Six waves per SIMD: caps the kernel at 80 VGPRs (91 without the hint, five

@@ -178,7 +178,7 @@ __device__ __forceinline__ void bend_bead(const SysDev &P, const Bufs &B, size_t
 // (one term for the single-routine entry points, all for the fused step).
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,
-                                               double *__restrict__ partial, double *__restrict__ f_heads) {
+                                               double *__restrict__ partial, double *__restrict__ f_heads, int vir) {
     __shared__ double red[BLK];
     const int s = __builtin_amdgcn_readfirstlane(blk_sys[blockIdx.x]);
     MFA_CUR(B, s);
@@ -190,6 +190,7 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     const bool store_f = !FUSED || *B.store_frc != 0;
     const bool sums = (MODE & M_BENDF) ? *B.store_frc != 0 : store_f;
     double e_wall = 0.0, e_fene = 0.0, e_bend = 0.0, e_bendm = 0.0, ke = 0.0;
+    double wb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
         // no contraction: the kernel's variants (with and without the
         // bending code) then round the walls, sums and kinetic energy alike,
@@ -209,7 +210,10 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
         const int m = nm > 0 ? ol % nm : 0;
         const int g0 = g - m;
         if (MODE & (M_WALL | M_FUSED)) wall_bead(P, ri, ti, F, e_wall);
-        if (MODE & (M_FENE | M_FUSED)) fene_bead(P, B, cur_off, ri, g, m, nm, F, e_fene);
+        if (MODE & (M_FENE | M_FUSED)) {
+            if (vir) fene_bead<true>(P, B, cur_off, ri, g, m, nm, F, e_fene, wb);
+            else fene_bead(P, B, cur_off, ri, g, m, nm, F, e_fene);
+        }
         constexpr bool FBEND = (FUSED && !(MODE & M_NOBEND)) || (MODE & M_BENDF) != 0;
         if ((MODE & M_BEND) || (FBEND && P.bending)) {
             if (brush) bend_bead(P, B, cur_off, g0, nm, m, ri, F, e_bend);
@@ -265,7 +269,14 @@ __global__ __launch_bounds__(BLK) void k_bonded(const SysDev *__restrict__ sys, 
     double *out = partial + (size_t)blockIdx.x * NPART;
     if (sums) {
         if (MODE & (M_WALL | M_FUSED)) { double t = block_sum(e_wall, red); if (threadIdx.x == 0) out[C_VWALL] = t; }
-        if (MODE & (M_FENE | M_FUSED)) { double t = block_sum(e_fene, red); if (threadIdx.x == 0) out[C_VINTRA] = t; }
+        if (MODE & (M_FENE | M_FUSED)) {
+            double t = block_sum(e_fene, red);
+            if (threadIdx.x == 0) out[C_VINTRA] = t;
+            for (int q = 0; q < 6; ++q) {       // the FENE bond virial (zero when off)
+                double bq = vir ? block_sum(wb[q], red) : 0.0;
+                if (threadIdx.x == 0) out[C_WBXX + q] = bq;
+            }
+        }
         if ((MODE & (M_BEND | M_BENDF)) || FUSED) { double t = block_sum(e_bend, red); if (threadIdx.x == 0) out[C_VBEND] = t; }
         if ((MODE & (M_BENDM | M_BENDF)) || FUSED) { double t = block_sum(e_bendm, red); if (threadIdx.x == 0) out[C_VBENDM] = t; }
         if (MODE & (M_VV2 | M_FUSED)) {

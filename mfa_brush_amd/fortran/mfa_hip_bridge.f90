@@ -37,7 +37,8 @@ module mfa_hip_iface
     end type
 
     type, bind(C) :: mfa_energies
-        real(c_double) :: v_fluid_fluid, v_fluid_wall, v_intra_molec, v_bend, v_bend_melt, t_fluid, virial(6)
+        real(c_double) :: v_fluid_fluid, v_fluid_wall, v_intra_molec, v_bend, v_bend_melt, t_fluid, virial(6), &
+                          virial_bond(6)
     end type
 
     interface

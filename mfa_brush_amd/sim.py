@@ -183,8 +183,9 @@ class Simulation:
     def energies(self, isys: int = 0) -> dict:
         e = _lib.Energies()
         self._c(self.lib.mfa_hip_get_energies(self.h, isys, C.byref(e)), "get_energies")
-        d = {k: getattr(e, k) for k, _ in _lib.Energies._fields_ if k != "virial"}
+        d = {k: getattr(e, k) for k, _ in _lib.Energies._fields_ if not k.startswith("virial")}
         d["virial"] = np.array(list(e.virial))
+        d["virial_bond"] = np.array(list(e.virial_bond))
         return d
 
     def f_on_heads(self, isys: int = 0):

@@ -628,7 +628,7 @@ double orc_wall_wall(const orc_params *p, double sigma_w) {
 }
 
 static double fene_chain_set(const orc_params *p, const double *r0, double *force, int first,
-                             int n_mon, int n_chain, int *n_str) {
+                             int n_mon, int n_chain, int *n_str, double *W) {
     double v = 0.0;
     const double clamp = F2D(0.98);
     for (int c = 0; c < n_chain; ++c)
@@ -644,10 +644,15 @@ static double fene_chain_set(const orc_params *p, const double *r0, double *forc
             if (q > clamp) { if (n_str) ++*n_str; q = clamp; }
             v = v + (-0.5 * log(1.0 - q));
             double rd = p->k_chain / (1.0 - q) * p->inv_r_chain_2;
+            double fl[3];
             for (int d = 0; d < 3; ++d) {
-                double fl = rd * dr[d];
-                force[3 * i + d] -= fl;
-                force[3 * j + d] += fl;
+                fl[d] = rd * dr[d];
+                force[3 * i + d] -= fl[d];
+                force[3 * j + d] += fl[d];
+            }
+            if (W) {    /* r_ij (x) F_ij, F_ij = -fl the force on i (intra_molec.f90:115-120) */
+                W[0] -= fl[0] * dr[0]; W[1] -= fl[1] * dr[1]; W[2] -= fl[2] * dr[2];
+                W[3] -= fl[0] * dr[1]; W[4] -= fl[0] * dr[2]; W[5] -= fl[1] * dr[2];
             }
         }
     return p->k_chain * v;
@@ -656,9 +661,21 @@ static double fene_chain_set(const orc_params *p, const double *r0, double *forc
 /* intra_molec, intra_molec.f90:12-129,266: FENE brush + melt bonds */
 double orc_intra_molec(const orc_params *p, const double *r0, double *force, int *n_stretched) {
     if (n_stretched) *n_stretched = 0;
-    double vb = fene_chain_set(p, r0, force, 0, p->n_mon, p->n_chain, n_stretched);
-    double vm = fene_chain_set(p, r0, force, p->n_mon * p->n_chain, p->n_mon_d, p->n_chain_d, n_stretched);
+    double vb = fene_chain_set(p, r0, force, 0, p->n_mon, p->n_chain, n_stretched, NULL);
+    double vm = fene_chain_set(p, r0, force, p->n_mon * p->n_chain, p->n_mon_d, p->n_chain_d, n_stretched, NULL);
     return vb + vm;
+}
+
+/* FENE bond virial W = sum over the bonds of brush and melt chains of
+ * r_ij (x) F_ij (xx yy zz xy xz yz), r_ij = r_i - r_j the minimum image in x
+ * and y and F_ij the FENE force on i, as intra_molec.f90:115-120 accumulates
+ * it for the melt (its brush loop, :61-66, adds the opposite sign). */
+void orc_bond_virial(const orc_params *p, const double *r0, double W[6]) {
+    double *tmp = (double *)calloc(3 * (size_t)(p->n_part > 0 ? p->n_part : 1), sizeof(double));
+    for (int q = 0; q < 6; ++q) W[q] = 0.0;
+    fene_chain_set(p, r0, tmp, 0, p->n_mon, p->n_chain, NULL, W);
+    fene_chain_set(p, r0, tmp, p->n_mon * p->n_chain, p->n_mon_d, p->n_chain_d, NULL, W);
+    free(tmp);
 }
 
 /* bending(1), bending.f90:19-121 (melt=0) / bending_melt(1),

@@ -99,6 +99,7 @@ void orc_pair_virial(const orc_params *p, const double *r0, const double *v, con
 double orc_fluid_wall(const orc_params *p, const double *r0, const int *a_type, double *force);
 double orc_wall_wall(const orc_params *p, double sigma_w);
 double orc_intra_molec(const orc_params *p, const double *r0, double *force, int *n_stretched);
+void orc_bond_virial(const orc_params *p, const double *r0, double W[6]);
 double orc_bending(const orc_params *p, const double *r0, double *force, int melt);
 void orc_constant_force(const orc_params *p, const double *mass, const int *a_type, double *force);
 

@@ -83,6 +83,7 @@ def load(omp: bool = False) -> C.CDLL:
     lib.orc_pair_stats_cells.restype = i
     lib.orc_fluid_fluid_brute.argtypes = [vp, vp, vp, vp, d, u64, i, vp, vp]
     lib.orc_pair_virial.argtypes = [vp, vp, vp, vp, d, u64, i, vp]
+    lib.orc_bond_virial.argtypes = [vp, vp, vp]
     lib.orc_fluid_wall.argtypes = [vp, vp, vp, vp]
     lib.orc_fluid_wall.restype = d
     lib.orc_wall_wall.argtypes = [vp, d]
@@ -195,6 +196,12 @@ class Oracle:
         self.lib.orc_pair_virial(self.po, _p(np.ascontiguousarray(r0, float)), _p(np.ascontiguousarray(v, float)),
                                  _p(np.ascontiguousarray(a_type, np.int32)), self.lib.orc_r2_min(self.po, r_time),
                                  step, noise, _p(w))
+        return w
+
+    def bond_virial(self, r0):
+        """sum over FENE bonds of r_ij (x) F_ij: xx yy zz xy xz yz."""
+        w = np.zeros(6)
+        self.lib.orc_bond_virial(self.po, _p(np.ascontiguousarray(r0, float)), _p(w))
         return w
 
     def check_fluid_fluid(self, r0, a_type, r_2_min=0.0):

@@ -1,4 +1,4 @@
-"""GPU parity of the virial, the DPD range beyond the LJ range, the force
+"""GPU parity of the pair and bond virial, the DPD range beyond the LJ range, the force
 carried through a rebuild, and the mixed path's force error at full size.
 
 Bars: FP64 within 1e-10 of the oracle (relative to the largest diagonal
@@ -59,6 +59,55 @@ def test_dpd_beyond_lj_range(oracle_mod, precision):
     fo, _ = oracle_mod.Oracle(p).fluid_fluid(r0, v, t, step=3, noise=1)
     err = np.linalg.norm(f - fo, axis=1).max() / np.linalg.norm(fo, axis=1).max()
     assert err < (1e-11 if precision == "fp64" else 1e-5), err
+
+
+def _wclose(w, want, rtol):
+    scale = np.abs(want[:3]).max()
+    assert scale > 0.0
+    err = np.abs(np.asarray(w) - np.asarray(want)).max() / scale
+    assert err < rtol, (err, w, want)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_bond_virial_vs_oracle(oracle_mod, precision):
+    """The FENE bond part of the virial against the oracle's sum over bonds
+    of r_ij (x) F_ij, and the whole virial (pairs + bonds) against the
+    oracle's two parts: routine path (fluid_fluid, then intra_molec), fused
+    steps (the pair kernel's bonded epilogue; forces evaluated at the
+    positions the last step leaves), and the DPD-VV step (bonds in the
+    bonded kernel).  The bond part is FP64 on both paths."""
+    p, t, r0, v = _perturbed(configs.channel_c1, thermostat=False, rc_dpd=1.0)
+    o = oracle_mod.Oracle(p)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    with Simulation(p, precision=precision) as s:
+        s.set_virial(True)
+        s.set_state(r0, v, t)
+        s.fluid_fluid()
+        s.intra_molec()
+        e = s.energies()
+        wbo = o.bond_virial(r0)
+        _wclose(e["virial_bond"], wbo, 1e-11)
+        _wclose(e["virial"], o.pair_virial(r0, v, t, noise=0) + wbo, tol)
+        s.fluid_fluid()                                  # a pair evaluation clears the bond part
+        assert not s.energies()["virial_bond"].any()
+        s.md_steps(3)
+        e = s.energies()
+        r = s.get_state()["r0"]
+        wbo = o.bond_virial(r)
+        _wclose(e["virial_bond"], wbo, 1e-11)
+        _wclose(e["virial"], o.pair_virial(r, v, t, noise=0) + wbo, tol)
+        s.set_virial(False)
+        s.md_steps(1)
+        e = s.energies()
+        assert not e["virial"].any() and not e["virial_bond"].any()
+    p2, t2, r2, v2 = _dpd_vv_c1(1.0)
+    with Simulation(p2, precision=precision) as s:
+        s.set_virial(True)
+        s.set_state(r2, v2, t2)
+        s.md_steps(3)
+        e = s.energies()
+        r = s.get_state()["r0"]
+    _wclose(e["virial_bond"], oracle_mod.Oracle(p2).bond_virial(r), 1e-11)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -30,11 +30,39 @@ def test_library_exports_every_declared_symbol():
     assert lib.mfa_hip_version() == 1
 
 
-def test_struct_layout_matches_header():
-    # offsets of the trailing fields pin the packing of mfa_system_params
-    assert _lib.SystemParams.t_idx0.offset == _lib.SystemParams.seed.offset + 8
-    assert C.sizeof(_lib.SystemParams) % 8 == 0
-    assert C.sizeof(_lib.Energies) == 12 * 8
+def test_struct_layout_matches_header(tmp_path):
+    """Sizes and field offsets of the ctypes structures equal what a C
+    compiler gives for include/mfa_hip.h."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = {"mfa_system_params": ("SystemParams", [f for f, _ in _lib.SystemParams._fields_]),
+              "mfa_energies": ("Energies", [f for f, _ in _lib.Energies._fields_]),
+              "mfa_flags": ("Flags", [f for f, _ in _lib.Flags._fields_])}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mfa_hip.h"', "int main(void) {"]
+    for cname, (_, names) in fields.items():
+        lines.append('printf("%%s size %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f in names:
+            lines.append('printf("%%s %%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    seen = 0
+    for line in out:
+        parts = line.split()
+        if len(parts) == 3 and parts[1] == "size":
+            assert C.sizeof(getattr(_lib, fields[parts[0]][0])) == int(parts[2]), line
+            seen += 1
+        elif len(parts) == 3:
+            st = getattr(_lib, fields[parts[0]][0])
+            assert getattr(st, parts[1]).offset == int(parts[2]), line
+            seen += 1
+    assert seen == sum(1 + len(n) for _, n in fields.values())
 
 
 def test_make_binning_host_matches_reference_formula():

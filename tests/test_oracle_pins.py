@@ -386,3 +386,72 @@ def test_dpd_vv_force_is_pair_force_at_new_velocities(oracle_mod, rc_dpd):
     _, v2, _, f2, _, _ = oracle_mod.Oracle(q).md_steps(r0, v, a0, t, step, 1, noise=1)
     assert np.array_equal(v2, v1)                    # DPD-VV leaves the velocities alone
     assert np.abs(f2 - f1).max() > 1e-3 * np.abs(want).max()
+
+
+# ------------------------------------------------ FENE bond virial ----
+def test_bond_virial_is_energy_derivative_under_strain(oracle_mod):
+    """Bond part of the virial: W_aa = -dV_bond/d(lambda_a) under r_a ->
+    lambda_a r_a (x and y with the box), and the shears likewise, V_bond the
+    oracle's FENE energy of brush and melt chains (some bonds crossing the
+    periodic x boundary)."""
+    p, o, r0, t = _melt_system(oracle_mod)
+    r0 = r0.copy()
+    r0[:, 0] = (r0[:, 0] + 10.0) % 12.0                 # melt chains now wrap in x
+    w = o.bond_virial(r0)
+    h = 1e-6
+
+    def energy(r, bx):
+        return oracle_mod.Oracle(p.replace(boundary=tuple(bx))).intra_molec(r)[1]
+
+    def d_scale(a):
+        out = []
+        for s in (1 + h, 1 - h):
+            r = r0.copy(); r[:, a] *= s
+            bx = list(p.boundary)
+            if a < 2:
+                bx[a] *= s
+            out.append(energy(r, bx))
+        return (out[0] - out[1]) / (2 * h)
+
+    def d_shear(a, b):
+        out = []
+        for g in (h, -h):
+            r = r0.copy(); r[:, a] += g * r0[:, b]
+            out.append(energy(r, p.boundary))
+        return (out[0] - out[1]) / (2 * h)
+
+    want = [-d_scale(0), -d_scale(1), -d_scale(2), None, -d_shear(0, 2), -d_shear(1, 2)]
+    scale = np.abs(w[:3]).max()
+    assert scale > 1.0
+    for q in (0, 1, 2, 4, 5):                           # z is not periodic: the xz and yz shears are exact
+        assert abs(w[q] - want[q]) < 1e-6 * scale, (q, w, want)
+
+
+def test_bond_virial_cluster_identity(oracle_mod):
+    """sum_bonds r_ij (x) F_ij = sum_i r_i (x) F_i for chains that cross no
+    periodic boundary, F the oracle's FENE force: the bond virial uses the
+    force intra_molec adds, all six components."""
+    p, o, r0, t = _melt_system(oracle_mod)
+    r0 = r0.copy()
+    r0[:, :2] = r0[:, :2] * 0.5 + 3.0                     # squeeze into the box interior
+    f, _, _ = o.intra_molec(r0)
+    assert np.abs(f).max() > 1.0
+    want = [(r0[:, a] * f[:, b]).sum() for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    w = o.bond_virial(r0)
+    assert np.allclose(w, want, rtol=0, atol=1e-11 * np.abs(want).max()), (w, want)
+    assert abs(w[3]) > 1e-3 * np.abs(w[:3]).max()
+
+
+def test_bond_virial_single_bond_closed_form(oracle_mod):
+    """One FENE bond along d: W = -k d (x) d / (R0^2 (1 - |d|^2/R0^2)), the
+    reference's force factor k / (1 - q) / R0^2 (intra_molec.f90:46) times the
+    separation, k the table value k_chain (already times R0^2)."""
+    p = configs.channel_params((12.0, 12.0, 16.0), 2, 1, 0, thermostat=False)
+    o = oracle_mod.Oracle(p)
+    d = np.array([0.6, -0.5, 0.4])
+    r0 = np.array([[6.0, 6.0, 8.0], [6.0, 6.0, 8.0]]) + np.array([[0.0, 0.0, 0.0], d])
+    q = d.dot(d) * p.inv_r_chain_2
+    rd = p.k_chain / (1.0 - q) * p.inv_r_chain_2
+    dij = -d                                              # r_0 - r_1
+    want = [-rd * dij[a] * dij[b] for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    assert np.allclose(o.bond_virial(r0), want, rtol=1e-12)
