@@ -205,14 +205,17 @@ int  mfa_hip_get_cells(mfa_ctx *ctx, int32_t isys, int32_t *n_cells, int32_t *ce
                        int32_t *cell_count, int32_t *cell_beads);
 
 /* ---- timing ------------------------------------------------------------
- * Mode 1: each fused-step kernel group (vv1, rebuild, pair, bonded) is
- * bracketed by hipEvents on the context's stream.  Mode 2: only the pair
- * kernel is (two event records per step instead of eight).  Mode 0: off.
- * While timing, md_steps launches eagerly.  Totals are in milliseconds. */
+ * Mode 1: each fused-step kernel group is bracketed by hipEvents on the
+ * context's stream: "vv1" (the first half kick, once per md_steps call when
+ * the step uses the fused integrator), "rebuild", "pair", "bonded" (bending
+ * if any, then the integrator; the bonded kernel on the routine path) and
+ * "dpd_vv".  Mode 2: only the pair kernel is.  Mode 0: off.  While timing,
+ * md_steps launches eagerly.  Totals are in milliseconds. */
 int  mfa_hip_set_timing(mfa_ctx *ctx, int32_t on);
 int  mfa_hip_get_kernel_time(mfa_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
-int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair virial (default off); waits
-                                                        for the context's stream and drops the captured graphs */
+int  mfa_hip_set_virial(mfa_ctx *ctx, int32_t on);   /* accumulate the FP64 pair and bond virial (default off);
+                                                        waits for the context's stream and drops the captured
+                                                        graphs */
 
 /* Verlet-list capacity (entries per bead) of the pair kernels: -1 automatic
  * (default), 0 no list (the pair kernels scan the 27 or 125 neighbour cells
