@@ -39,10 +39,14 @@ int  mfa_mlp_forward(mfa_mlp *m, const float *x, int32_t rows, float *y);
 int  mfa_mlp_gradient(mfa_mlp *m, const float *x, const float *t, int32_t rows, float *loss, float *grad);
 /* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch`
  * rows drawn without replacement per epoch (splitmix64(seed) shuffles);
- * loss_history[steps] (optional) receives each minibatch MSE */
+ * loss_history[steps] (optional) receives each minibatch MSE.  One step is
+ * captured as a hipGraph and replayed `steps` times (see mfa_mlp_set_graph) */
 int  mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, int32_t steps, int32_t batch,
                    float lr, uint64_t seed, float *loss_history);
-/* C (M x N) = A (M x K) B (K x N) through the BF16 MFMA tile (operands
+/* training by graph replay (on, the default) or by launching each step's
+ * kernels (off); both give the same bits.  No reference counterpart */
+int  mfa_mlp_set_graph(mfa_mlp *m, int32_t on);
+/* C (M x N) = A (M x K) B (K x N) through the BF16 MFMA GEMM (operands
  * rounded to BF16, FP32 accumulation); exposes the tile for tests */
 int  mfa_mlp_gemm_check(const float *a, const float *b, int32_t M, int32_t N, int32_t K, float *c);
 const char *mfa_mlp_last_error(void);

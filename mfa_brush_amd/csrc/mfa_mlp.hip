@@ -6,17 +6,24 @@
 // W_1, b_1, ...  Activations are FP32 in memory and rounded to BF16 (nearest
 // even) when a GEMM loads them, so every product runs on the matrix cores.
 //
-// GEMM tile: one 64-lane wave computes a 32x32 block of C with
-// v_mfma_f32_32x32x8_bf16 (K = 8 per instruction).  Lane l holds
-//   A: rows i = l % 32, columns k = 4 (l / 32) + j, j = 0..3
-//   B: rows k = 4 (l / 32) + j, columns n = l % 32
+// GEMM: a block of four 64-lane waves computes a 64x64 tile of C; the
+// operands are staged per K chunk of 32 in LDS as BF16 (rounded once, nearest
+// even), A row-major and B transposed, so each wave reads its fragments with
+// 16-byte LDS loads and issues v_mfma_f32_32x32x16_bf16 (gfx950, K = 16 per
+// instruction) on its 32x32 quarter.  Lane l of a wave holds
+//   A: row i = l % 32, columns k = 8 (l / 32) + j, j = 0..7
+//   B: column n = l % 32, rows k = 8 (l / 32) + j
 //   C: column n = l % 32, rows i = 8 (r / 4) + 4 (l / 32) + r % 4, r = 0..15
 // (checked on the device by tests/test_gpu_mlp.py against exact integer
-// products).  Strides give the transposes the backward pass needs.
+// products).  Strides give the transposes the backward pass needs; the
+// staging loop maps consecutive threads along whichever operand dimension
+// has unit stride, so the global loads coalesce either way.  The forward
+// pass adds the bias and the SiLU in the GEMM's epilogue.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -25,58 +32,119 @@
 
 namespace mlp {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet)
-__device__ __forceinline__ short bf16_rne(float f) {
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40u);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
     u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
+    return (unsigned short)(u >> 16);
 }
 
-// C[i, n] (+)= sum_k A(i, k) B(k, n);  A(i, k) = A[i*sai + k*sak], B(k, n) = B[k*sbk + n*sbn]
-__global__ __launch_bounds__(64) void k_gemm(const float *__restrict__ A, long sai, long sak,
-                                            const float *__restrict__ Bm, long sbk, long sbn,
-                                            float *__restrict__ C, long ldc, int M, int N, int K, int accumulate) {
-    const int l = threadIdx.x;
-    const int i0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
-    const int ia = i0 + (l & 31), nb = n0 + (l & 31), kq = 4 * (l >> 5);
+__device__ __forceinline__ float silu(float z);
+
+constexpr int GT = 64;          // block tile of C (rows and columns)
+constexpr int KC = 32;          // K chunk staged in LDS
+constexpr int KP = KC + 8;      // LDS row length in BF16 elements (80 bytes: 16-byte aligned rows)
+
+// Epilogue of the GEMM: plain store, accumulate into C, or z = acc + b stored
+// in C with h = z (EPI_BIAS) or h = SiLU(z) (EPI_SILU) stored in H.
+enum Epi { EPI_STORE = 0, EPI_ACC, EPI_BIAS, EPI_SILU };
+
+__device__ __forceinline__ float apply_epi(float v, float *c, int epi, const float *bias, float *h, int n) {
+    if (epi == EPI_ACC) v += *c;
+    if (epi >= EPI_BIAS) {
+        v += bias[n];
+        *h = epi == EPI_SILU ? silu(v) : v;
+    }
+    return v;
+}
+
+// C[i, n] (op)= sum_k A(i, k) B(k, n);  A(i, k) = A[i*sai + k*sak], B(k, n) = B[k*sbk + n*sbn].
+// Split K: block z of gridDim.z takes k in [z kslice, (z + 1) kslice) and, when
+// gridDim.z > 1, writes its raw partial tile to ws + z M N (row-major, ld N)
+// for k_splitk_reduce.  The next chunk's operands are loaded into registers
+// while the current chunk's MFMAs run.
+__global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, long sai, long sak,
+                                             const float *__restrict__ Bm, long sbk, long sbn,
+                                             float *__restrict__ C, long ldc, int M, int N, int K, int epi,
+                                             const float *__restrict__ bias, float *__restrict__ H,
+                                             int kslice, float *__restrict__ ws) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[GT * KP];     // As[i][k]
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[GT * KP];     // Bs[n][k]
+    constexpr int PER = GT * KC / 256;                                       // staged elements per thread
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+    const int kb = blockIdx.z * kslice, ke = min(K, kb + kslice);
+    const int wi = 32 * (w & 1), wn = 32 * (w >> 1);
+    const bool a_k_fast = sak == 1, b_n_fast = sbn == 1;
+    float ra[PER], rb[PER];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = t + 256 * q;
+            const int r = a_k_fast ? e / KC : e % GT, ka = a_k_fast ? e % KC : e / GT;
+            const int gi = i0 + r, gka = k0 + ka;
+            ra[q] = (gi < M && gka < ke) ? A[(long)gi * sai + (long)gka * sak] : 0.0f;
+            const int n = b_n_fast ? e % GT : e / KC, kbb = b_n_fast ? e / GT : e % KC;
+            const int gn = n0 + n, gkb = k0 + kbb;
+            rb[q] = (gn < N && gkb < ke) ? Bm[(long)gkb * sbk + (long)gn * sbn] : 0.0f;
+        }
+    };
     float16_t acc = {};
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        short4_t a, b;
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + kq + j;
-            a[j] = (ia < M && k < K) ? bf16_rne(A[(long)ia * sai + (long)k * sak]) : (short)0;
-            b[j] = (nb < N && k < K) ? bf16_rne(Bm[(long)k * sbk + (long)nb * sbn]) : (short)0;
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = t + 256 * q;
+            const int r = a_k_fast ? e / KC : e % GT, ka = a_k_fast ? e % KC : e / GT;
+            As[r * KP + ka] = bf16_rne(ra[q]);
+            const int n = b_n_fast ? e % GT : e / KC, kbb = b_n_fast ? e / GT : e % KC;
+            Bs[n * KP + kbb] = bf16_rne(rb[q]);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        __syncthreads();
+        if (k0 + KC < ke) load(k0 + KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 16) {
+            const bf16x8 a = *(const bf16x8 *)&As[(wi + (l & 31)) * KP + kk + 8 * (l >> 5)];
+            const bf16x8 b = *(const bf16x8 *)&Bs[(wn + (l & 31)) * KP + kk + 8 * (l >> 5)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
     }
+    const int n = n0 + wn + (l & 31);
     for (int r = 0; r < 16; ++r) {
-        const int i = i0 + 8 * (r >> 2) + kq + (r & 3);
-        const int n = n0 + (l & 31);
+        const int i = i0 + wi + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
         if (i < M && n < N) {
-            float *c = C + (long)i * ldc + n;
-            *c = accumulate ? *c + acc[r] : acc[r];
+            if (gridDim.z > 1) {
+                ws[(long)blockIdx.z * M * N + (long)i * N + n] = acc[r];
+            } else {
+                float *c = C + (long)i * ldc + n;
+                *c = apply_epi(acc[r], c, epi, bias, H + (long)i * ldc + n, n);
+            }
         }
     }
+}
+
+// C (op)= sum over z = 0 .. S-1, in order, of the split-K partials
+__global__ __launch_bounds__(256) void k_splitk_reduce(const float *__restrict__ ws, int S, int M, int N,
+                                                      float *__restrict__ C, long ldc, int epi,
+                                                      const float *__restrict__ bias, float *__restrict__ H) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)M * N) return;
+    const int i = (int)(e / N), n = (int)(e % N);
+    float v = 0.0f;
+    for (int z = 0; z < S; ++z) v += ws[(long)z * M * N + e];
+    float *c = C + (long)i * ldc + n;
+    *c = apply_epi(v, c, epi, bias, H + (long)i * ldc + n, n);
 }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.0f / (1.0f + expf(-z));
     return s * (1.0f + z * (1.0f - s));
-}
-
-// z += b (row-wise bias); h = SiLU(z) for hidden layers, h = z for the output
-__global__ __launch_bounds__(256) void k_bias_act(float *__restrict__ z, float *__restrict__ h,
-                                                 const float *__restrict__ b, int M, int N, int act) {
-    const long t = (long)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (long)M * N) return;
-    const float v = z[t] + b[t % N];
-    z[t] = v;
-    h[t] = act ? silu(v) : v;
 }
 
 // dz = dh * SiLU'(z)
@@ -105,17 +173,39 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ y, const 
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// db[n] = sum_i dz[i, n] (fixed order per column)
+// db[n] = sum_i dz[i, n]: a block takes 32 columns, its 8 row groups sum
+// rows g, g + 8, ... (four running sums, so four loads are in flight) and the
+// groups' sums are added in order (deterministic)
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ dz, float *__restrict__ db, int M, int N) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float sh[8][32];
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int n = blockIdx.x * 32 + c;
     float s = 0.f;
-    for (int i = 0; i < M; ++i) s += dz[(long)i * N + n];
-    db[n] = s;
+    if (n < N) {
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        int i = g;
+        for (; i + 24 < M; i += 32) {
+            s0 += dz[(long)i * N + n];
+            s1 += dz[(long)(i + 8) * N + n];
+            s2 += dz[(long)(i + 16) * N + n];
+            s3 += dz[(long)(i + 24) * N + n];
+        }
+        for (; i < M; i += 8) s0 += dz[(long)i * N + n];
+        s = (s0 + s1) + (s2 + s3);
+    }
+    sh[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) {
+        float v = sh[0][c];
+        for (int q = 1; q < 8; ++q) v += sh[q][c];
+        db[n] = v;
+    }
 }
 
-// sum of the per-block partials (one block)
-__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ part, int n, float *__restrict__ out) {
+// sum of the per-block partials (one block), into out[*slot] (out[0] when
+// slot is null)
+__global__ __launch_bounds__(256) void k_sum(const float *__restrict__ part, int n, float *__restrict__ out,
+                                            const int *__restrict__ slot) {
     __shared__ float sh[256];
     float s = 0.f;
     for (int i = threadIdx.x; i < n; i += 256) s += part[i];
@@ -125,15 +215,20 @@ __global__ __launch_bounds__(256) void k_sum(const float *__restrict__ part, int
         if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (threadIdx.x == 0) out[slot ? *slot : 0] = sh[0];
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master parameters
+// Adam (Kingma & Ba 2015) on the FP32 master parameters, at Adam step
+// t0 + *step + 1 (the bias corrections computed on the device, so a captured
+// training step replays unchanged)
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
                                              float *__restrict__ m2, long n, float lr, float b1, float b2, float eps,
-                                             float c1, float c2) {
+                                             const int *__restrict__ step, long t0) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
+    const float ts = (float)(t0 + *step + 1);
+    const float c1 = 1.0f / (1.0f - powf(b1, ts));
+    const float c2 = 1.0f / (1.0f - powf(b2, ts));
     const float gt = g[t];
     const float a = b1 * m1[t] + (1.0f - b1) * gt;
     const float b = b2 * m2[t] + (1.0f - b2) * gt * gt;
@@ -141,13 +236,16 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float
     p[t] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
 }
 
-// gather rows idx[] of x (n_in) and t (n_out) into the batch buffers
+// the training loop's step counter
+__global__ void k_step_inc(int *step) { *step += 1; }
+
+// gather rows idx[*step * m + r] of x (n_in) and t (n_out) into the batch buffers
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ x, const float *__restrict__ t,
-                                               const int *__restrict__ idx, int m, int n_in, int n_out,
-                                               float *__restrict__ xb, float *__restrict__ tb) {
+                                               const int *__restrict__ idx, const int *__restrict__ step, int m,
+                                               int n_in, int n_out, float *__restrict__ xb, float *__restrict__ tb) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= m) return;
-    const long s = idx[r];
+    const long s = idx[(long)*step * m + r];
     for (int q = 0; q < n_in; ++q) xb[(long)r * n_in + q] = x[s * n_in + q];
     for (int q = 0; q < n_out; ++q) tb[(long)r * n_out + q] = t[s * n_out + q];
 }
@@ -171,15 +269,40 @@ struct mfa_mlp {
     int cap = 0;
     std::vector<float *> d_h, d_z;      // h_l (input of layer l, l = 0..L) and z_l (pre-activation, l = 0..L-1)
     float *d_dy = nullptr, *d_da = nullptr, *d_db = nullptr, *d_t = nullptr, *d_part = nullptr, *d_loss = nullptr;
+    float *d_ws = nullptr;              // split-K workspace of the batch's GEMMs
+    int *d_step = nullptr;              // the training loop's step counter
+    bool use_graph = true;              // train by replaying one captured step
 };
+
+static int blocks(long n) { return (int)((n + 255) / 256); }
+
+// Split-K plan of a GEMM: with fewer than 64 output tiles and at least 4 K
+// chunks, the K range is split over up to 16 blocks per tile (each at least
+// two chunks), so a small weight-gradient GEMM with a long K fills more CUs.
+// Returns the number of slices S and their length.
+static int split_plan(int M, int N, int K, int *kslice) {
+    const int tiles = ((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+    const int chunks = (K + KC - 1) / KC;
+    int S = 1;
+    if (tiles < 64 && chunks >= 4) S = std::max(1, std::min({chunks / 2, (128 + tiles - 1) / tiles, 16}));
+    const int per = (chunks + S - 1) / S;
+    *kslice = per * KC;
+    return (chunks + per - 1) / per;
+}
+
+static long split_ws(int M, int N, int K) {
+    int ks;
+    const int S = split_plan(M, N, K, &ks);
+    return S > 1 ? (long)S * M * N : 0;
+}
 
 static void free_batch(mfa_mlp *m) {
     for (float *p : m->d_h) if (p) (void)hipFree(p);
     for (float *p : m->d_z) if (p) (void)hipFree(p);
     m->d_h.clear(); m->d_z.clear();
-    float *bufs[] = {m->d_dy, m->d_da, m->d_db, m->d_t, m->d_part, m->d_loss};
+    float *bufs[] = {m->d_dy, m->d_da, m->d_db, m->d_t, m->d_part, m->d_loss, m->d_ws};
     for (float *p : bufs) if (p) (void)hipFree(p);
-    m->d_dy = m->d_da = m->d_db = m->d_t = m->d_part = m->d_loss = nullptr;
+    m->d_dy = m->d_da = m->d_db = m->d_t = m->d_part = m->d_loss = m->d_ws = nullptr;
     m->cap = 0;
 }
 
@@ -200,26 +323,38 @@ static int ensure_batch(mfa_mlp *m, int rows) {
     const long nblk = ((long)rows * m->w[L] + 255) / 256;
     HIPCHK(hipMalloc((void **)&m->d_part, sizeof(float) * (size_t)std::max(1L, nblk)));
     HIPCHK(hipMalloc((void **)&m->d_loss, sizeof(float)));
+    long ws = 1;
+    for (int l = 0; l < L; ++l) {
+        const int ni = m->w[l], no = m->w[l + 1];
+        ws = std::max({ws, split_ws(rows, no, ni), split_ws(ni, no, rows), split_ws(rows, ni, no)});
+    }
+    HIPCHK(hipMalloc((void **)&m->d_ws, sizeof(float) * (size_t)ws));
     m->cap = rows;
     return 0;
 }
 
-static void gemm(mfa_mlp *m, const float *A, long sai, long sak, const float *B, long sbk, long sbn, float *C, long ldc,
-                 int M, int N, int K, int acc) {
-    dim3 g((M + 31) / 32, (N + 31) / 32);
-    hipLaunchKernelGGL(k_gemm, g, dim3(64), 0, m->st, A, sai, sak, B, sbk, sbn, C, ldc, M, N, K, acc);
+// C (op)= A B; ws: the split-K workspace (at least split_ws floats), or null
+// for one pass over K
+static void gemm(hipStream_t st, const float *A, long sai, long sak, const float *B, long sbk, long sbn, float *C,
+                 long ldc, int M, int N, int K, int epi, float *ws, const float *bias = nullptr, float *H = nullptr) {
+    int ks = K;
+    const int S = ws ? split_plan(M, N, K, &ks) : 1;
+    if (S == 1) ks = K;
+    dim3 g((M + GT - 1) / GT, (N + GT - 1) / GT, S);
+    hipLaunchKernelGGL(k_gemm, g, dim3(256), 0, st, A, sai, sak, B, sbk, sbn, C, ldc, M, N, K, epi, bias, H, ks, ws);
+    if (S > 1)
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks((long)M * N)), dim3(256), 0, st, (const float *)ws, S, M, N, C,
+                           ldc, epi, bias, H);
 }
 
-static int blocks(long n) { return (int)((n + 255) / 256); }
 
 // forward of rows already in d_h[0]; output in d_h[L]
 static void forward(mfa_mlp *m, int rows) {
     const int L = (int)m->w.size() - 1;
     for (int l = 0; l < L; ++l) {
         const int ni = m->w[l], no = m->w[l + 1];
-        gemm(m, m->d_h[l], ni, 1, m->d_p + m->woff[l], no, 1, m->d_z[l], no, rows, no, ni, 0);
-        hipLaunchKernelGGL(k_bias_act, dim3(blocks((long)rows * no)), dim3(256), 0, m->st, m->d_z[l], m->d_h[l + 1],
-                           m->d_p + m->boff[l], rows, no, l < L - 1 ? 1 : 0);
+        gemm(m->st, m->d_h[l], ni, 1, m->d_p + m->woff[l], no, 1, m->d_z[l], no, rows, no, ni,
+             l < L - 1 ? EPI_SILU : EPI_BIAS, m->d_ws, m->d_p + m->boff[l], m->d_h[l + 1]);
     }
 }
 
@@ -229,17 +364,17 @@ static void backward(mfa_mlp *m, int rows) {
     const long nout = (long)rows * m->w[L];
     hipLaunchKernelGGL(k_mse, dim3(blocks(nout)), dim3(256), 0, m->st, m->d_h[L], m->d_t, m->d_dy, m->d_part, nout,
                        1.0f / (float)nout);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, m->st, m->d_part, blocks(nout), m->d_loss);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, m->st, m->d_part, blocks(nout), m->d_loss, (const int *)nullptr);
     float *dz = m->d_dy;            // dL/dz of the current layer (rows x no)
     for (int l = L - 1; l >= 0; --l) {
         const int ni = m->w[l], no = m->w[l + 1];
         // dW_l = h_l^T dz  (ni x no), K = rows
-        gemm(m, m->d_h[l], 1, ni, dz, no, 1, m->d_g + m->woff[l], no, ni, no, rows, 0);
-        hipLaunchKernelGGL(k_colsum, dim3(blocks(no)), dim3(256), 0, m->st, dz, m->d_g + m->boff[l], rows, no);
+        gemm(m->st, m->d_h[l], 1, ni, dz, no, 1, m->d_g + m->woff[l], no, ni, no, rows, EPI_STORE, m->d_ws);
+        hipLaunchKernelGGL(k_colsum, dim3((no + 31) / 32), dim3(256), 0, m->st, dz, m->d_g + m->boff[l], rows, no);
         if (l == 0) break;
         // dh_l = dz W_l^T  (rows x ni), then through SiLU of layer l-1
         float *dh = dz == m->d_da ? m->d_db : m->d_da;
-        gemm(m, dz, no, 1, m->d_p + m->woff[l], 1, no, dh, ni, rows, ni, no, 0);
+        gemm(m->st, dz, no, 1, m->d_p + m->woff[l], 1, no, dh, ni, rows, ni, no, EPI_STORE, m->d_ws);
         hipLaunchKernelGGL(k_act_back, dim3(blocks((long)rows * ni)), dim3(256), 0, m->st, dh, m->d_z[l - 1],
                            (long)rows * ni);
         dz = dh;
@@ -281,6 +416,9 @@ int mfa_mlp_create(const int32_t *widths, int32_t n_widths, uint64_t seed, mfa_m
         hipMalloc((void **)&m->d_m1, b) != hipSuccess || hipMalloc((void **)&m->d_m2, b) != hipSuccess) {
         mfa_mlp_destroy(m); return fail(MFA_MLP_E_HIP, "parameter allocation");
     }
+    if (hipMalloc((void **)&m->d_step, sizeof(int)) != hipSuccess) {
+        mfa_mlp_destroy(m); return fail(MFA_MLP_E_HIP, "step counter allocation");
+    }
     if (hipMemcpy(m->d_p, p.data(), b, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(m->d_m1, 0, b) != hipSuccess || hipMemset(m->d_m2, 0, b) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
@@ -296,6 +434,7 @@ int mfa_mlp_destroy(mfa_mlp *m) {
     free_batch(m);
     float *bufs[] = {m->d_p, m->d_g, m->d_m1, m->d_m2};
     for (float *p : bufs) if (p) (void)hipFree(p);
+    if (m->d_step) (void)hipFree(m->d_step);
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
     return 0;
@@ -395,19 +534,45 @@ int mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, in
     HIPCHK(hipMemcpy(d_ix, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const long nout = (long)bt * m->w[L];
-    for (int it = 0; it < steps; ++it) {
-        hipLaunchKernelGGL(k_gather, dim3(blocks(bt)), dim3(256), 0, m->st, (const float *)d_x, (const float *)d_t,
-                           (const int *)(d_ix + (size_t)it * bt), bt, m->w[0], m->w[L], m->d_h[0], m->d_t);
+    const long t0 = m->adam_t;
+    const int zero = 0;
+    HIPCHK(hipMemcpy(m->d_step, &zero, sizeof(int), hipMemcpyHostToDevice));
+    // one training step: minibatch gather, forward, backward, loss into the
+    // history at the step counter, Adam, counter + 1; every per-step value is
+    // read from the counter on the device, so the step replays as a graph
+    auto one_step = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_gather, dim3(blocks(bt)), dim3(256), 0, st, (const float *)d_x, (const float *)d_t,
+                           (const int *)d_ix, (const int *)m->d_step, bt, m->w[0], m->w[L], m->d_h[0], m->d_t);
         forward(m, bt);
         backward(m, bt);
-        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, m->st, m->d_part, blocks(nout), d_lh + it);
-        m->adam_t += 1;
-        const float c1 = 1.0f / (1.0f - std::pow(b1, (float)m->adam_t));
-        const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->adam_t));
-        hipLaunchKernelGGL(k_adam, dim3(blocks(m->n_par)), dim3(256), 0, m->st, m->d_p, m->d_g, m->d_m1, m->d_m2,
-                           m->n_par, lr, b1, b2, eps, c1, c2);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, m->d_part, blocks(nout), d_lh, (const int *)m->d_step);
+        hipLaunchKernelGGL(k_adam, dim3(blocks(m->n_par)), dim3(256), 0, st, m->d_p, m->d_g, m->d_m1, m->d_m2,
+                           m->n_par, lr, b1, b2, eps, (const int *)m->d_step, t0);
+        hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, st, m->d_step);
+    };
+    hipError_t e = hipSuccess;
+    if (m->use_graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        e = hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            one_step(m->st);
+            e = hipStreamEndCapture(m->st, &g);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        for (int it = 0; it < steps && e == hipSuccess; ++it) e = hipGraphLaunch(ge, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    } else {
+        for (int it = 0; it < steps; ++it) one_step(m->st);
     }
-    hipError_t e = hipGetLastError();
+    m->adam_t += steps;
+    if (e != hipSuccess) {
+        (void)hipFree(d_x); (void)hipFree(d_t); (void)hipFree(d_ix); (void)hipFree(d_lh);
+        return fail(MFA_MLP_E_HIP, std::string("train: ") + hipGetErrorString(e));
+    }
+    e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(m->st);
     std::vector<float> lh(steps);
     if (e == hipSuccess) e = hipMemcpy(lh.data(), d_lh, sizeof(float) * (size_t)steps, hipMemcpyDeviceToHost);
@@ -415,6 +580,12 @@ int mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, in
     if (e != hipSuccess) return fail(MFA_MLP_E_HIP, std::string("train: ") + hipGetErrorString(e));
     if (loss_history)
         for (int it = 0; it < steps; ++it) loss_history[it] = lh[it] / (float)nout;
+    return 0;
+}
+
+int mfa_mlp_set_graph(mfa_mlp *m, int32_t on) {
+    if (!m) return fail(MFA_MLP_E_ARG, "set_graph: null");
+    m->use_graph = on != 0;
     return 0;
 }
 
@@ -427,13 +598,15 @@ int mfa_mlp_gemm_check(const float *a, const float *b, int32_t M, int32_t N, int
     HIPCHK(hipMalloc((void **)&dc, sizeof(float) * (size_t)M * N));
     HIPCHK(hipMemcpy(da, a, sizeof(float) * (size_t)M * K, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(db, b, sizeof(float) * (size_t)K * N, hipMemcpyHostToDevice));
-    dim3 g((M + 31) / 32, (N + 31) / 32);
-    hipLaunchKernelGGL(k_gemm, g, dim3(64), 0, 0, (const float *)da, (long)K, 1L, (const float *)db, (long)N, 1L, dc,
-                       (long)N, M, N, K, 0);
+    float *dw = nullptr;
+    const long nws = split_ws(M, N, K);
+    if (nws > 0) HIPCHK(hipMalloc((void **)&dw, sizeof(float) * (size_t)nws));
+    gemm(0, da, (long)K, 1L, db, (long)N, 1L, dc, (long)N, M, N, K, EPI_STORE, dw);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(c, dc, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
     (void)hipFree(da); (void)hipFree(db); (void)hipFree(dc);
+    if (dw) (void)hipFree(dw);
     return 0;
 }
 

@@ -20,6 +20,7 @@ SIGNATURES = {
     "mfa_mlp_forward": [_vp, _vp, _i32, _vp],
     "mfa_mlp_gradient": [_vp, _vp, _vp, _i32, _vp, _vp],
     "mfa_mlp_train": [_vp, _vp, _vp, _i64, _i32, _i32, _f, C.c_uint64, _vp],
+    "mfa_mlp_set_graph": [_vp, _i32],
     "mfa_mlp_gemm_check": [_vp, _vp, _i32, _i32, _i32, _vp],
     "mfa_mlp_last_error": [],
 }
@@ -111,6 +112,11 @@ class Mlp:
         loss = C.c_float()
         _check(self.lib.mfa_mlp_gradient(self.h, _p(x), _p(t), x.shape[0], C.byref(loss), _p(g)), "gradient")
         return loss.value, g
+
+    def set_graph(self, on: bool = True):
+        """Train by replaying one captured step (default) or by launching each
+        step's kernels; the same bits either way."""
+        _check(self.lib.mfa_mlp_set_graph(self.h, 1 if on else 0), "set_graph")
 
     def train(self, x, t, steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
         x = _f32(x, (-1, self.widths[0]))

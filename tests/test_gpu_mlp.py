@@ -20,7 +20,8 @@ def _mlp(widths=WIDTHS, seed=3):
     return Mlp(widths, seed=seed)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 8), (70, 45, 19), (128, 128, 128), (5, 130, 300)])
+@pytest.mark.parametrize("shape", [(32, 32, 8), (70, 45, 19), (128, 128, 128), (5, 130, 300), (64, 64, 16),
+                                   (200, 150, 70), (129, 65, 33)])
 def test_mfma_tile_layout_exact_on_integers(shape):
     """Small integers are exact in BF16 and their products sum exactly in
     FP32, so the MFMA tile must reproduce A @ B bit for bit; a wrong lane
@@ -78,3 +79,25 @@ def test_training_fits_a_smooth_map():
     e1 = ((y1 - target(xt)) ** 2).mean()
     assert np.all(np.isfinite(hist)) and hist[-50:].mean() < hist[:50].mean() / 30
     assert e1 < e0 / 100 and e1 < 2e-3, (e0, e1)
+
+
+def test_graph_training_equals_eager():
+    """Training by replaying one captured step gives the bits of launching
+    every step's kernels: parameters and loss history after each of two
+    train calls, with forward passes (eager work) between them."""
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, size=(3000, 4)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0] * x[:, 3]), x[:, 1] * x[:, 2]], 1).astype(np.float32)
+    runs = []
+    for graph in (True, False):
+        out = []
+        with _mlp(seed=4) as m:
+            m.set_graph(graph)
+            for call in range(2):
+                h = m.train(x, t, steps=120, batch=256, lr=2e-3, seed=5 + call)
+                y = m.forward(x[:64])
+                out.append((h, m.params(), y))
+        runs.append(out)
+    for (hg, pg, yg), (he, pe, ye) in zip(*runs):
+        assert np.array_equal(hg, he) and np.array_equal(pg, pe) and np.array_equal(yg, ye)
+    assert runs[0][1][0][-10:].mean() < runs[0][0][0][:10].mean()
