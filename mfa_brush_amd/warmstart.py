@@ -131,32 +131,85 @@ def predict(mlp: Mlp, spec: Spec):
     return zc, y[:, 0], y[:, 1]
 
 
-def equilibrium_profiles(specs, equil_steps: int, avg_steps: int, every: int = 100, skin: float = 0.3):
-    """Run all channels in one batched context; profiles averaged over the
-    last avg_steps steps.  Returns a list of (brush, solvent) densities."""
-    built = []
-    for i, s in enumerate(specs):
-        built.append(build(s, None, skin))
-        print("  built training channel %d/%d (%d beads)" % (i + 1, len(specs), built[-1][0].n_part), flush=True)
-    with Simulation([b[0] for b in built]) as sim:
-        for i, (p, t, r0, v) in enumerate(built):
-            sim.set_state(r0, v, t, isys=i)
-        done = 0
-        while done < equil_steps:
-            n = min(5000, equil_steps - done)
-            sim.md_steps(n, graph=True)
-            done += n
-            print("  equilibrate %d/%d" % (done, equil_steps), flush=True)
-        sim.profiles_reset(N_BINS)
-        for q in range(avg_steps // every):
-            sim.md_steps(every, graph=True)
-            sim.profiles_accumulate()
-        out = []
-        for i, (p, *_r) in enumerate(built):
-            n_obs, dens, _, _ = sim.profiles(i)
-            out.append(densities(p, dens, n_obs))
-        fl = sim.flags()
-    return out, fl, int(sum(b[0].n_part for b in built))
+def _build_plain(args):
+    spec, skin = args
+    return build(spec, None, skin)
+
+
+def build_many(specs, skin: float = 0.3, workers: int = 16):
+    """Plain starts of many channels; from 64 channels on, built by a pool of
+    fresh worker processes (spawned, so they share nothing with a process
+    that already uses the GPU)."""
+    if len(specs) < 64 or workers <= 1:
+        return [build(s, None, skin) for s in specs]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    n = max(1, min(workers, os.cpu_count() or 1))
+    out = []
+    with ProcessPoolExecutor(max_workers=n, mp_context=mp.get_context("spawn")) as ex:
+        for c0 in range(0, len(specs), 1024):
+            part = specs[c0:c0 + 1024]
+            out.extend(ex.map(_build_plain, [(s, skin) for s in part], chunksize=16))
+            print("  built %d/%d training channels" % (len(out), len(specs)), flush=True)
+    return out
+
+
+# slots per batched context: the Verlet-list entries address slots below 2^23
+MAX_SLOTS = 7_000_000
+
+
+def _groups(built, max_slots: int = MAX_SLOTS):
+    """Consecutive groups of channels whose 256-aligned slot ranges fit one
+    context."""
+    groups, cur, slots = [], [], 0
+    for b in built:
+        n = (b[0].n_part + 255) // 256 * 256
+        if cur and slots + n > max_slots:
+            groups.append(cur)
+            cur, slots = [], 0
+        cur.append(b)
+        slots += n
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def equilibrium_profiles(specs, equil_steps: int, avg_steps: int, every: int = 100, skin: float = 0.3,
+                         workers: int = 16):
+    """Run the channels batched (as many per context as its slot range
+    holds); profiles averaged over the last avg_steps steps.  Returns the
+    list of (brush, solvent) densities, the summed anomaly flags and the bead
+    count."""
+    built = build_many(specs, skin, workers)
+    n_beads = int(sum(b[0].n_part for b in built))
+    print("  built %d training channels (%d beads)" % (len(built), n_beads), flush=True)
+    out, flags = [], {}
+    groups = _groups(built)
+    for gi, group in enumerate(groups):
+        with Simulation([b[0] for b in group]) as sim:
+            for i, (p, t, r0, v) in enumerate(group):
+                sim.set_state(r0, v, t, isys=i)
+            done = 0
+            while done < equil_steps:
+                n = min(1000, equil_steps - done)
+                sim.md_steps(n, graph=True)
+                done += n
+                if done % 5000 == 0 or done == equil_steps:
+                    print("  batch %d/%d (%d channels): equilibrate %d/%d" % (gi + 1, len(groups), len(group), done,
+                                                                            equil_steps), flush=True)
+            sim.profiles_reset(N_BINS)
+            for q in range(avg_steps // every):
+                sim.md_steps(every, graph=True)
+                sim.profiles_accumulate()
+                if (q + 1) * every % 5000 == 0:
+                    print("  batch %d/%d: average %d/%d" % (gi + 1, len(groups), (q + 1) * every, avg_steps),
+                          flush=True)
+            for i, (p, *_r) in enumerate(group):
+                n_obs, dens, _, _ = sim.profiles(i)
+                out.append(densities(p, dens, n_obs))
+            for k, v in sim.flags().items():
+                flags[k] = flags.get(k, 0) + int(v)
+    return out, flags, n_beads
 
 
 def training_specs(n: int, seed: int = 11):
@@ -168,12 +221,12 @@ def training_specs(n: int, seed: int = 11):
     return specs
 
 
-def train(specs, profiles, steps: int = 3000, seed: int = 5):
+def train(specs, profiles, steps: int = 3000, seed: int = 5, batch: int = 512):
     zc, _ = _bins()
     X = np.concatenate([features(s, zc) for s in specs])
     T = np.concatenate([np.stack([pr[0], pr[1]], 1) for pr in profiles]).astype(np.float32)
     mlp = Mlp(WIDTHS, seed=seed)
-    hist = mlp.train(X, T, steps=steps, batch=512, lr=2e-3, seed=seed)
+    hist = mlp.train(X, T, steps=steps, batch=batch, lr=2e-3, seed=seed)
     return mlp, hist
 
 
@@ -217,7 +270,7 @@ START_NAMES = ("plain", "generated", "warm")
 
 
 def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 100, block: int = 1000,
-                mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21):
+                mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21, batch: int = 512):
     """The whole warm-start method at a bounded size (tens of seconds on one
     GPU): n_train small channels (12 sigma boxes, N in {8, 16}) run batched
     for equil + avg steps to give training profiles, the surrogate trains on
@@ -234,7 +287,9 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
                   eps_bs=float(rng.uniform(0.5, 1.5)), side=12.0, seed=300 + i) for i in range(n_train)]
     profs, _, n_beads = equilibrium_profiles(specs, equil, avg)
     t_sim = time.perf_counter() - t0
-    mlp, hist = train(specs, profs, steps=mlp_steps)
+    t_train = time.perf_counter()
+    mlp, hist = train(specs, profs, steps=mlp_steps, batch=batch)
+    t_train = time.perf_counter() - t_train
     held = held or Spec(0.25, 16, 1.0, side=20.0, seed=907)
     starts = [build(held, None), build_generated(held), build(held, mlp)]
     runs, n_part, secs, fl = block_profiles_starts(starts, n_blocks, block=block)
@@ -243,6 +298,7 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
     zc, rb, rs = predict(mlp, held)
     out = {"held_out": asdict(held), "n_part": n_part, "block_steps": block, "blocks": n_blocks,
            "train_channels": n_train, "train_beads": n_beads, "train_steps": equil + avg,
+           "mlp_steps": mlp_steps, "mlp_batch": batch, "mlp_seconds": t_train,
            "mlp_loss_first_last": [float(hist[:20].mean()), float(hist[-20:].mean())],
            "surrogate_l1": l1_error(np.stack([rb, rs]), ref),
            "late_gap_l1": max(l1_error(a, b) for a in late for b in late),
@@ -261,14 +317,15 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
 
 
 def main_tte(argv):
-    """--tte report.json [n_train equil avg n_blocks side sigma_g n_mon eps_bs]:
-    one bounded_tte run, written as JSON."""
+    """--tte report.json [n_train equil avg n_blocks side sigma_g n_mon eps_bs
+    mlp_steps batch]: one bounded_tte run, written as JSON."""
     out_path = argv[0]
     vals = [float(x) for x in argv[1:]]
-    d = [24, 20000, 10000, 100, 20.0, 0.25, 16, 1.0]
+    d = [24, 20000, 10000, 100, 20.0, 0.25, 16, 1.0, 2000, 512]
     vals = vals + d[len(vals):]
     held = Spec(vals[5], int(vals[6]), vals[7], side=vals[4], seed=907)
-    r = bounded_tte(n_train=int(vals[0]), equil=int(vals[1]), avg=int(vals[2]), n_blocks=int(vals[3]), held=held)
+    r = bounded_tte(n_train=int(vals[0]), equil=int(vals[1]), avg=int(vals[2]), n_blocks=int(vals[3]), held=held,
+                    mlp_steps=int(vals[8]), batch=int(vals[9]))
     print(json.dumps(r), flush=True)
     with open(out_path, "w") as f:
         json.dump(r, f, indent=1)
