@@ -613,18 +613,21 @@ __device__ __forceinline__ Sep32 sep32(float4 pf, float4 pj, float ox, float oy,
 enum T32Field { F_REACH = 0, F_LO, F_HI, F_EPS, F_SIG2, F_ESH, F_DPD, F_IRC, F_NF };
 
 // LJ part of candidate pj (slot m) for bead k, and whether the pair takes a
-// DPD term: 0 none, 1 within the LJ range, 2 beyond it (DPD_CUT_OFF only).
-// ix: the candidate's table index.  The early exit compares r^2 with the
-// reach, max(LJ range widened by 2^-14, DPD range widened by 1e-4).
+// DPD term, with in_lj whether it lies within the LJ range (a DPD pair
+// beyond it: DPD_CUT_OFF only).  Two flags rather than one three-valued
+// result, so both stay lane masks.  ix: the candidate's table index.  The
+// early exit compares r^2 with the reach, max(LJ range widened by 2^-14, DPD
+// range widened by 1e-4).
 template <bool DPD, bool VIR>
-__device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const Pair32Ctx &X,
-                                         const double4 *__restrict__ pos, int k, int ti, float4 pf,
-                                         float ox, float oy, float oz, float4 pj, int m, int ix, bool calc_e,
-                                         PairAcc &a) {
+__device__ __forceinline__ bool pair32_lj(const SysDev &P, const float *T, const Pair32Ctx &X,
+                                          const double4 *__restrict__ pos, int k, int ti, float4 pf,
+                                          float ox, float oy, float oz, float4 pj, int m, int ix, bool calc_e,
+                                          PairAcc &a, bool &in_lj) {
 #pragma clang fp contract(off)
     const Sep32 d = sep32(pf, pj, ox, oy, oz);
     float r2 = d.r2;
-    if (!(r2 < T[F_REACH * 32 + ix])) return 0;
+    in_lj = false;
+    if (!(r2 < T[F_REACH * 32 + ix])) return false;
     bool lj = r2 < T[F_LO * 32 + ix];
     if (!lj && r2 < T[F_HI * 32 + ix]) {
         // borderline: decide with the FP64 reference arithmetic
@@ -654,11 +657,11 @@ __device__ __forceinline__ int pair32_lj(const SysDev &P, const float *T, const 
             a.w[3] += (double)(d.x * fy); a.w[4] += (double)(d.x * fz); a.w[5] += (double)(d.y * fz);
         }
     }
-    if (!DPD) return 0;
+    in_lj = lj;
+    if (!DPD) return false;
     // DPD_CUT_OFF: the F_DPD entry is the DPD range^2 for non-head pairs and
     // -1 for pairs with a head (or a channel without the DPD thermostat)
-    const bool do_dpd = X.dpd_cut ? (r2 < T[F_DPD * 32 + ix]) : lj;
-    return do_dpd ? (lj ? 1 : 2) : 0;
+    return X.dpd_cut ? (r2 < T[F_DPD * 32 + ix]) : lj;
 }
 
 // DPD part of a pair that pair32_lj flagged (dpd_forces.f90:45-115).  lj: the
@@ -787,22 +790,23 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
         // full (the partner's identity word reloaded with its position)
         auto visit = [&](int m, int code, float4 pj, int ix) {
             const float4 o = off[code];
-            const int f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, calc_e, a);
+            bool in_lj;
+            const bool f = pair32_lj<DPD, VIR>(P, T, X, pos, k, ti, pf, o.x, o.y, o.z, pj, m, ix, calc_e, a, in_lj);
             if (DPD && f) {
                 if (dq < dcap) {
                     myq[dq * BLK] = m;
-                    myc[dq * BLK] = (unsigned short)(code | (f == 1 ? 0x80 : 0));
+                    myc[dq * BLK] = (unsigned short)(code | (in_lj ? 0x80 : 0));
                     ++dq;
                 } else {
-                    pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, posf[m], velf[m], f == 1, ix, a);
+                    pair32_dpd<VIR>(T, X, oi, pf, vi, o.x, o.y, o.z, posf[m], velf[m], in_lj, ix, a);
                 }
             }
         };
         const int nn = B.nl_cap > 0 ? B.nl_n[k] : -1;
         if (nn >= 0 && nn <= B.nl_cap) {
-            // list entries are slots below n_slots (k_nlist takes them from
-            // checked cell ranges); the clamp only keeps a gather in bounds
-            const unsigned last = (unsigned)B.n_slots - 1u;
+            // list entries are slots below n_slots: k_nlist takes them from
+            // checked cell ranges and writes the unused entries of a bead's
+            // partial last group as zeros, so every entry gathered is in range
             // one 16-byte group of four entries, their four gathers in
             // flight together; TAIL: the list's last, partial group.  The
             // partner's type comes with the entry, so only its three cell-
@@ -820,7 +824,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
                 float3 pj[NL_UNROLL];
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
-                    mm[u] = min((unsigned)ent[u] >> NL_SLOT_SHIFT, last);
+                    mm[u] = (unsigned)ent[u] >> NL_SLOT_SHIFT;
                     pj[u] = *(const float3 *)((const char *)posf + mm[u] * (unsigned)sizeof(float4));
                 }
                 if (!decltype(tail)::value)
