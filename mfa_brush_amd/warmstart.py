@@ -175,11 +175,13 @@ def _groups(built, max_slots: int = MAX_SLOTS):
 
 
 def equilibrium_profiles(specs, equil_steps: int, avg_steps: int, every: int = 100, skin: float = 0.3,
-                         workers: int = 16):
+                         workers: int = 16, halves: list | None = None):
     """Run the channels batched (as many per context as its slot range
     holds); profiles averaged over the last avg_steps steps.  Returns the
     list of (brush, solvent) densities, the summed anomaly flags and the bead
-    count."""
+    count.  With `halves` (a list), the two halves of the averaging window
+    are averaged separately and their pairs appended to it; the profile
+    returned is then their mean (equal sample counts)."""
     built = build_many(specs, skin, workers)
     n_beads = int(sum(b[0].n_part for b in built))
     print("  built %d training channels (%d beads)" % (len(built), n_beads), flush=True)
@@ -197,16 +199,25 @@ def equilibrium_profiles(specs, equil_steps: int, avg_steps: int, every: int = 1
                 if done % 5000 == 0 or done == equil_steps:
                     print("  batch %d/%d (%d channels): equilibrate %d/%d" % (gi + 1, len(groups), len(group), done,
                                                                             equil_steps), flush=True)
-            sim.profiles_reset(N_BINS)
-            for q in range(avg_steps // every):
-                sim.md_steps(every, graph=True)
-                sim.profiles_accumulate()
-                if (q + 1) * every % 5000 == 0:
-                    print("  batch %d/%d: average %d/%d" % (gi + 1, len(groups), (q + 1) * every, avg_steps),
-                          flush=True)
-            for i, (p, *_r) in enumerate(group):
-                n_obs, dens, _, _ = sim.profiles(i)
-                out.append(densities(p, dens, n_obs))
+            n_avg = avg_steps // every
+            parts = [n_avg // 2, n_avg - n_avg // 2] if halves is not None else [n_avg]
+            got = []
+            q = 0
+            for part in parts:
+                sim.profiles_reset(N_BINS)
+                for _ in range(part):
+                    sim.md_steps(every, graph=True)
+                    sim.profiles_accumulate()
+                    q += 1
+                    if q * every % 5000 == 0:
+                        print("  batch %d/%d: average %d/%d" % (gi + 1, len(groups), q * every, avg_steps),
+                              flush=True)
+                got.append([densities(p, sim.profiles(i)[1], sim.profiles(i)[0]) for i, (p, *_r) in enumerate(group)])
+            if halves is None:
+                out.extend(got[0])
+            else:
+                halves.extend(zip(got[0], got[1]))
+                out.extend(0.5 * (a + b) for a, b in zip(got[0], got[1]))
             for k, v in sim.flags().items():
                 flags[k] = flags.get(k, 0) + int(v)
     return out, flags, n_beads
@@ -316,6 +327,80 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
     return out
 
 
+def learning_curve(n_train: int = 1024, n_held: int = 128, equil: int = 30000, avg: int = 10000,
+                   sizes=(16, 32, 64, 128, 256, 512, 1024), mlp_steps: int = 4000, batch: int = 512,
+                   side: float = 12.0, seed: int = 41):
+    """What the surrogate gains from its training data: n_train + n_held
+    channels (box side `side`, grafting density U[0.05, 0.5], N in {8, 16},
+    eps_bs U[0.5, 1.5]) run batched to equilibrium profiles; the surrogate is
+    trained on the first n channels for each n in sizes and scored on the
+    n_held held-out channels by the relative L1 distance of its predicted
+    (brush, solvent) profiles from their simulated ones.  The score of the
+    lattice start's own profile (the uniform solvent and straight chains a
+    plain start places) is given beside it as the no-surrogate baseline."""
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    specs = [Spec(sigma_g=float(rng.uniform(0.05, 0.5)), n_mon=int(rng.choice([8, 16])),
+                  eps_bs=float(rng.uniform(0.5, 1.5)), side=side, seed=5000 + i) for i in range(n_train + n_held)]
+    halves = []
+    profs, flags, n_beads = equilibrium_profiles(specs, equil, avg, halves=halves)
+    t_sim = time.perf_counter() - t0
+    # noise of a simulated profile: with independent halves A and B of the
+    # averaging window, their mean M lies about |A - B| / 2 = |A - M| from
+    # the noiseless profile, so L1(A, M) estimates how far even an exact
+    # surrogate would score from the held-out profiles
+    noise = [l1_error(a, 0.5 * (a + b)) for a, b in halves[n_train:]]
+    train_specs, held_specs = specs[:n_train], specs[n_train:]
+    train_profs, held_profs = profs[:n_train], profs[n_train:]
+    zc, dz = _bins()
+
+    def lattice_profile(spec):
+        # the plain start's own density profile: straight chains (one bead
+        # per sigma from the wall) and uniform solvent filling the rest
+        p, _, r0, _ = build(spec, None)
+        area = p.boundary[0] * p.boundary[1]
+        n_brush = p.n_chain * p.n_mon            # brush beads come first, as the device profiles count them
+        h_b = np.histogram(r0[:n_brush, 2], bins=N_BINS, range=(0.0, p.boundary[2]))[0]
+        h_s = np.histogram(r0[n_brush:, 2], bins=N_BINS, range=(0.0, p.boundary[2]))[0]
+        rho = np.stack([h_b, h_s]) / (area * dz)
+        return 0.5 * (rho + rho[:, ::-1])
+
+    base = [l1_error(lattice_profile(s), pr) for s, pr in zip(held_specs, held_profs)]
+    curve = []
+    for n in sizes:
+        if n > n_train:
+            break
+        t1 = time.perf_counter()
+        mlp, hist = train(train_specs[:n], train_profs[:n], steps=mlp_steps, batch=batch)
+        errs = []
+        for s, pr in zip(held_specs, held_profs):
+            _, rb, rs = predict(mlp, s)
+            errs.append(l1_error(np.stack([rb, rs]), pr))
+        errs = np.array(errs)
+        curve.append({"n_train": n, "samples": n * N_BINS, "held_l1_mean": float(errs.mean()),
+                      "held_l1_median": float(np.median(errs)), "held_l1_p90": float(np.quantile(errs, 0.9)),
+                      "loss_last": float(hist[-20:].mean()), "seconds": time.perf_counter() - t1})
+        print(json.dumps(curve[-1]), flush=True)
+    return {"n_train": n_train, "n_held": n_held, "side": side, "equil_steps": equil, "avg_steps": avg,
+            "beads": n_beads, "sim_seconds": t_sim, "flags": flags, "mlp_steps": mlp_steps, "mlp_batch": batch,
+            "lattice_start_l1_mean": float(np.mean(base)), "lattice_start_l1_median": float(np.median(base)),
+            "held_noise_l1_mean": float(np.mean(noise)), "held_noise_l1_median": float(np.median(noise)),
+            "curve": curve, "seconds": time.perf_counter() - t0}
+
+
+def main_curve(argv):
+    """--curve report.json [n_train n_held equil avg mlp_steps]: learning_curve, written as JSON."""
+    out_path = argv[0]
+    d = [1024, 128, 30000, 10000, 4000]
+    vals = [int(x) for x in argv[1:]]
+    vals = vals + d[len(vals):]
+    sizes = tuple(n for n in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192) if n <= vals[0])
+    r = learning_curve(n_train=vals[0], n_held=vals[1], equil=vals[2], avg=vals[3], sizes=sizes, mlp_steps=vals[4])
+    print(json.dumps({k: v for k, v in r.items() if k != "curve"}), flush=True)
+    with open(out_path, "w") as f:
+        json.dump(r, f, indent=1)
+
+
 def main_tte(argv):
     """--tte report.json [n_train equil avg n_blocks side sigma_g n_mon eps_bs
     mlp_steps batch]: one bounded_tte run, written as JSON."""
@@ -342,6 +427,8 @@ def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     if argv and argv[0] == "--tte":
         return main_tte(argv[1:])
+    if argv and argv[0] == "--curve":
+        return main_curve(argv[1:])
     out_path = argv[0] if argv else "warmstart_report.json"
     n_train = int(argv[1]) if len(argv) > 1 else 48
     equil, avg = (int(argv[2]), int(argv[3])) if len(argv) > 3 else (40000, 20000)
