@@ -43,6 +43,10 @@ int  mfa_mlp_gradient(mfa_mlp *m, const float *x, const float *t, int32_t rows, 
  * captured as a hipGraph and replayed `steps` times (see mfa_mlp_set_graph) */
 int  mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, int32_t steps, int32_t batch,
                    float lr, uint64_t seed, float *loss_history);
+/* learning-rate schedule of later train calls: a cosine from lr at a call's
+ * first step to lr * lr_final at its last; lr_final = 1 (the default) keeps
+ * it constant.  No reference counterpart */
+int  mfa_mlp_set_lr_schedule(mfa_mlp *m, float lr_final);
 /* training by graph replay (on, the default) or by launching each step's
  * kernels (off); both give the same bits.  No reference counterpart */
 int  mfa_mlp_set_graph(mfa_mlp *m, int32_t on);

@@ -220,12 +220,18 @@ __global__ __launch_bounds__(256) void k_sum(const float *__restrict__ part, int
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters, at Adam step
 // t0 + *step + 1 (the bias corrections computed on the device, so a captured
-// training step replays unchanged)
+// training step replays unchanged).  With lr_final < 1 the learning rate
+// follows a cosine from lr at the call's first step to lr * lr_final at its
+// last (step *step of n_steps); lr_final == 1 keeps it constant.
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
                                              float *__restrict__ m2, long n, float lr, float b1, float b2, float eps,
-                                             const int *__restrict__ step, long t0) {
+                                             const int *__restrict__ step, long t0, int n_steps, float lr_final) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
+    if (lr_final != 1.0f) {
+        const float u = n_steps > 1 ? (float)*step / (float)(n_steps - 1) : 0.0f;
+        lr = lr * (lr_final + (1.0f - lr_final) * 0.5f * (1.0f + cosf(3.14159265f * u)));
+    }
     const float ts = (float)(t0 + *step + 1);
     const float c1 = 1.0f / (1.0f - powf(b1, ts));
     const float c2 = 1.0f / (1.0f - powf(b2, ts));
@@ -272,6 +278,7 @@ struct mfa_mlp {
     float *d_ws = nullptr;              // split-K workspace of the batch's GEMMs
     int *d_step = nullptr;              // the training loop's step counter
     bool use_graph = true;              // train by replaying one captured step
+    float lr_final = 1.0f;              // learning rate at a train call's last step, relative (1: constant)
 };
 
 static int blocks(long n) { return (int)((n + 255) / 256); }
@@ -547,7 +554,7 @@ int mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, in
         backward(m, bt);
         hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, m->d_part, blocks(nout), d_lh, (const int *)m->d_step);
         hipLaunchKernelGGL(k_adam, dim3(blocks(m->n_par)), dim3(256), 0, st, m->d_p, m->d_g, m->d_m1, m->d_m2,
-                           m->n_par, lr, b1, b2, eps, (const int *)m->d_step, t0);
+                           m->n_par, lr, b1, b2, eps, (const int *)m->d_step, t0, steps, m->lr_final);
         hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, st, m->d_step);
     };
     hipError_t e = hipSuccess;
@@ -580,6 +587,12 @@ int mfa_mlp_train(mfa_mlp *m, const float *x, const float *t, int64_t n_rows, in
     if (e != hipSuccess) return fail(MFA_MLP_E_HIP, std::string("train: ") + hipGetErrorString(e));
     if (loss_history)
         for (int it = 0; it < steps; ++it) loss_history[it] = lh[it] / (float)nout;
+    return 0;
+}
+
+int mfa_mlp_set_lr_schedule(mfa_mlp *m, float lr_final) {
+    if (!m || !(lr_final > 0.0f && lr_final <= 1.0f)) return fail(MFA_MLP_E_ARG, "set_lr_schedule: lr_final must be in (0, 1]");
+    m->lr_final = lr_final;
     return 0;
 }
 

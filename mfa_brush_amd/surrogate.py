@@ -21,6 +21,7 @@ SIGNATURES = {
     "mfa_mlp_gradient": [_vp, _vp, _vp, _i32, _vp, _vp],
     "mfa_mlp_train": [_vp, _vp, _vp, _i64, _i32, _i32, _f, C.c_uint64, _vp],
     "mfa_mlp_set_graph": [_vp, _i32],
+    "mfa_mlp_set_lr_schedule": [_vp, _f],
     "mfa_mlp_gemm_check": [_vp, _vp, _i32, _i32, _i32, _vp],
     "mfa_mlp_last_error": [],
 }
@@ -118,9 +119,13 @@ class Mlp:
         step's kernels; the same bits either way."""
         _check(self.lib.mfa_mlp_set_graph(self.h, 1 if on else 0), "set_graph")
 
-    def train(self, x, t, steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
+    def train(self, x, t, steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 7,
+              lr_final: float = 1.0) -> np.ndarray:
+        """Adam steps on minibatches of x -> t; lr_final < 1: cosine decay of
+        the learning rate to lr * lr_final over the steps of this call."""
         x = _f32(x, (-1, self.widths[0]))
         t = _f32(t, (-1, self.widths[-1]))
+        _check(self.lib.mfa_mlp_set_lr_schedule(self.h, C.c_float(lr_final)), "set_lr_schedule")
         hist = np.empty(max(steps, 1), np.float32)
         _check(self.lib.mfa_mlp_train(self.h, _p(x), _p(t), x.shape[0], steps, batch, C.c_float(lr),
                                       C.c_uint64(seed), _p(hist)), "train")

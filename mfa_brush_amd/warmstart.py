@@ -232,12 +232,14 @@ def training_specs(n: int, seed: int = 11):
     return specs
 
 
-def train(specs, profiles, steps: int = 3000, seed: int = 5, batch: int = 512):
+def train(specs, profiles, steps: int = 3000, seed: int = 5, batch: int = 512, lr_final: float = 0.02):
+    """The surrogate trained on the channels' profiles: Adam at 2e-3, decayed
+    by a cosine to lr_final of it over the steps (1: constant)."""
     zc, _ = _bins()
     X = np.concatenate([features(s, zc) for s in specs])
     T = np.concatenate([np.stack([pr[0], pr[1]], 1) for pr in profiles]).astype(np.float32)
     mlp = Mlp(WIDTHS, seed=seed)
-    hist = mlp.train(X, T, steps=steps, batch=batch, lr=2e-3, seed=seed)
+    hist = mlp.train(X, T, steps=steps, batch=batch, lr=2e-3, seed=seed, lr_final=lr_final)
     return mlp, hist
 
 
@@ -329,7 +331,7 @@ def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_block
 
 def learning_curve(n_train: int = 1024, n_held: int = 128, equil: int = 30000, avg: int = 10000,
                    sizes=(16, 32, 64, 128, 256, 512, 1024), mlp_steps: int = 4000, batch: int = 512,
-                   side: float = 12.0, seed: int = 41):
+                   side: float = 12.0, seed: int = 41, profiles_path: str | None = None):
     """What the surrogate gains from its training data: n_train + n_held
     channels (box side `side`, grafting density U[0.05, 0.5], N in {8, 16},
     eps_bs U[0.5, 1.5]) run batched to equilibrium profiles; the surrogate is
@@ -337,13 +339,27 @@ def learning_curve(n_train: int = 1024, n_held: int = 128, equil: int = 30000, a
     n_held held-out channels by the relative L1 distance of its predicted
     (brush, solvent) profiles from their simulated ones.  The score of the
     lattice start's own profile (the uniform solvent and straight chains a
-    plain start places) is given beside it as the no-surrogate baseline."""
+    plain start places) is given beside it as the no-surrogate baseline.
+    profiles_path (.npz): the simulated profiles are loaded from it when it
+    exists (the same channels: same counts, seed and run lengths), else
+    written to it, so the training can be repeated without the runs."""
     t0 = time.perf_counter()
     rng = np.random.default_rng(seed)
     specs = [Spec(sigma_g=float(rng.uniform(0.05, 0.5)), n_mon=int(rng.choice([8, 16])),
                   eps_bs=float(rng.uniform(0.5, 1.5)), side=side, seed=5000 + i) for i in range(n_train + n_held)]
     halves = []
-    profs, flags, n_beads = equilibrium_profiles(specs, equil, avg, halves=halves)
+    key = np.array([n_train, n_held, equil, avg, seed, side])
+    if profiles_path and os.path.exists(profiles_path):
+        with np.load(profiles_path) as z:
+            assert np.array_equal(z["key"], key), "profiles file is for other channels"
+            halves = list(zip(z["half_a"], z["half_b"]))
+            profs = [0.5 * (a + b) for a, b in halves]
+            flags, n_beads = {}, int(z["beads"])
+    else:
+        profs, flags, n_beads = equilibrium_profiles(specs, equil, avg, halves=halves)
+        if profiles_path:
+            np.savez(profiles_path, key=key, beads=n_beads, half_a=np.array([a for a, _ in halves]),
+                     half_b=np.array([b for _, b in halves]))
     t_sim = time.perf_counter() - t0
     # noise of a simulated profile: with independent halves A and B of the
     # averaging window, their mean M lies about |A - B| / 2 = |A - M| from
@@ -389,13 +405,15 @@ def learning_curve(n_train: int = 1024, n_held: int = 128, equil: int = 30000, a
 
 
 def main_curve(argv):
-    """--curve report.json [n_train n_held equil avg mlp_steps]: learning_curve, written as JSON."""
+    """--curve report.json [n_train n_held equil avg mlp_steps [profiles.npz]]: learning_curve, written as JSON."""
     out_path = argv[0]
     d = [1024, 128, 30000, 10000, 4000]
-    vals = [int(x) for x in argv[1:]]
+    vals = [int(x) for x in argv[1:6]]
     vals = vals + d[len(vals):]
+    prof = argv[6] if len(argv) > 6 else None
     sizes = tuple(n for n in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192) if n <= vals[0])
-    r = learning_curve(n_train=vals[0], n_held=vals[1], equil=vals[2], avg=vals[3], sizes=sizes, mlp_steps=vals[4])
+    r = learning_curve(n_train=vals[0], n_held=vals[1], equil=vals[2], avg=vals[3], sizes=sizes, mlp_steps=vals[4],
+                       profiles_path=prof)
     print(json.dumps({k: v for k, v in r.items() if k != "curve"}), flush=True)
     with open(out_path, "w") as f:
         json.dump(r, f, indent=1)

@@ -101,3 +101,27 @@ def test_graph_training_equals_eager():
     for (hg, pg, yg), (he, pe, ye) in zip(*runs):
         assert np.array_equal(hg, he) and np.array_equal(pg, pe) and np.array_equal(yg, ye)
     assert runs[0][1][0][-10:].mean() < runs[0][0][0][:10].mean()
+
+
+def test_lr_schedule_matches_stepwise_calls():
+    """A train call with the cosine learning-rate schedule equals one-step
+    calls at the rates the schedule gives, lr (f + (1 - f)(1 + cos(pi s /
+    (N - 1))) / 2) at step s, with Adam's moments and bias-correction count
+    carried across the calls (one training row, so every minibatch is the
+    same row)."""
+    from mfa_brush_amd.surrogate import Mlp
+    widths = [4, 32, 32, 2]
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(1, 4)).astype(np.float32)
+    t = rng.normal(size=(1, 2)).astype(np.float32)
+    n, lr, f = 12, 3e-3, 0.1
+    with Mlp(widths, seed=4) as a, Mlp(widths, seed=4) as b, Mlp(widths, seed=4) as c:
+        p0 = a.params()
+        a.train(x, t, steps=n, batch=1, lr=lr, lr_final=f)
+        for s in range(n):
+            b.train(x, t, steps=1, batch=1, lr=float(lr * (f + (1 - f) * 0.5 * (1 + np.cos(np.pi * s / (n - 1))))))
+        c.train(x, t, steps=n, batch=1, lr=lr)            # constant rate: a different end point
+        pa, pb, pc = a.params(), b.params(), c.params()
+    assert np.abs(pa - p0).max() > 5 * lr * f
+    assert np.abs(pa - pb).max() < 1e-6, np.abs(pa - pb).max()
+    assert np.abs(pa - pc).max() > 1e-4
