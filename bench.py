@@ -13,7 +13,7 @@ way and reported beside it.  The pair kernel's average launch duration comes
 from HIP events on the library's stream over the timed steps.
 
 time_to_equilibrium: a bounded run of the learned warm start (training
-profiles from 24 small batched channels, surrogate training, then a held-out
+profiles from 64 small batched channels, surrogate training, then a held-out
 12.8k-bead channel run for 100 blocks of 1000 steps from the plain and from
 the warm start side by side), blocks to the 10 / 5 / 2 % profile bands for
 both starts.

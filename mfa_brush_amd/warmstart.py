@@ -282,8 +282,8 @@ def time_to_equilibrium(blocks, ref, tol: float = 0.02, consecutive: int = 5):
 START_NAMES = ("plain", "generated", "warm")
 
 
-def bounded_tte(n_train: int = 24, equil: int = 20000, avg: int = 10000, n_blocks: int = 100, block: int = 1000,
-                mlp_steps: int = 2000, held: Spec | None = None, seed: int = 21, batch: int = 512):
+def bounded_tte(n_train: int = 64, equil: int = 20000, avg: int = 20000, n_blocks: int = 100, block: int = 1000,
+                mlp_steps: int = 6000, held: Spec | None = None, seed: int = 21, batch: int = 512):
     """The whole warm-start method at a bounded size (tens of seconds on one
     GPU): n_train small channels (12 sigma boxes, N in {8, 16}) run batched
     for equil + avg steps to give training profiles, the surrogate trains on
@@ -424,7 +424,7 @@ def main_tte(argv):
     mlp_steps batch]: one bounded_tte run, written as JSON."""
     out_path = argv[0]
     vals = [float(x) for x in argv[1:]]
-    d = [24, 20000, 10000, 100, 20.0, 0.25, 16, 1.0, 2000, 512]
+    d = [64, 20000, 20000, 100, 20.0, 0.25, 16, 1.0, 6000, 512]
     vals = vals + d[len(vals):]
     held = Spec(vals[5], int(vals[6]), vals[7], side=vals[4], seed=907)
     r = bounded_tte(n_train=int(vals[0]), equil=int(vals[1]), avg=int(vals[2]), n_blocks=int(vals[3]), held=held,

@@ -61,7 +61,7 @@ def test_c4_batch_pair_counts_momentum_and_single_runs(c4, oracle_mod):
 
 def test_warm_start_beats_plain_start():
     """The learned part: a surrogate trained on the batched simulator's
-    profiles (24 small channels) seeds a held-out channel whose first
+    profiles (64 small channels) seeds a held-out channel whose first
     block of MD is at least 2x closer (L1) to the equilibrium profiles than
     the first block of the plain straight-chain start and of the reference's
     generator start (ordered gen_brush brush, gen_droplet solvent)."""
