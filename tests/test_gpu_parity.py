@@ -541,6 +541,22 @@ def test_nve_drift_bounded():
     assert abs(r32["slope_per_1000_steps"] - r64["slope_per_1000_steps"]) < 2e-5
 
 
+def test_nve_drift_full_c2():
+    """Thermostat off, 1e4 steps of the full 998,784-bead brush channel
+    (the NVE check of that workload): the total energy stays within 5e-4 of
+    |E0| on both paths, and the mixed path's drift slope matches FP64's."""
+    from mfa_brush_amd.nve import drift
+    p, t, r0, v = configs.channel_c2(thermostat=False, rc_dpd=None, skin=0.4)
+    p = p.replace(flags=Flags(**{**p.flags.__dict__, "thermostat": False}), friction=0.0, sig=0.0)
+    r64 = drift(p, t, r0, v, "fp64", 10000, 500)
+    r32 = drift(p, t, r0, v, "mixed", 10000, 500)
+    print("C2 NVE: fp64 max %.2e slope %.2e; mixed max %.2e slope %.2e per 1000 steps"
+          % (r64["max_abs_rel"], r64["slope_per_1000_steps"], r32["max_abs_rel"], r32["slope_per_1000_steps"]))
+    assert r64["flags"]["nonfinite"] == 0 and r32["flags"]["nonfinite"] == 0
+    assert r64["max_abs_rel"] < 5e-4 and r32["max_abs_rel"] < 5e-4, (r64["max_abs_rel"], r32["max_abs_rel"])
+    assert abs(r32["slope_per_1000_steps"] - r64["slope_per_1000_steps"]) < 1e-5
+
+
 @pytest.mark.parametrize("system", ["brush_solvent", "brush_melt"])
 def test_chain_observables_vs_oracle(system):
     """Radius of gyration per axis of the brush and melt chains (unfolded
