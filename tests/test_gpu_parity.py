@@ -59,6 +59,25 @@ def test_try_fixture_golden_on_gpu(try_fixture, oracle_mod):
         assert s.pair_count() == co.sum() // 2
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_try_trajectory_follows_fort41(try_fixture, precision):
+    """From try/conf_old, 6 fused calls of 5 steps: V_ff, V_bond and V_wall
+    after each follow the reference's own try/fort.41 record (its ziggurat
+    noise against our Philox: V_ff within 0.5 %, bond and wall within 1e-4,
+    the oracle's bars in tests/test_oracle_pins.py)."""
+    from test_oracle_pins import FORT41_TOL, fort41_errors
+    p = try_fixture["p"]
+    ens = []
+    with Simulation(p, precision=precision) as s:
+        s.set_state(try_fixture["r0"], try_fixture["v"], try_fixture["a_type"], try_fixture["a"])
+        for _ in range(6):
+            s.md_steps(5)
+            ens.append(s.energies())
+    for i, err in enumerate(fort41_errors(ens, try_fixture["fort41"][:6])):
+        for k, tol in FORT41_TOL.items():
+            assert err[k] < tol, (5 * (i + 1), k, err[k])
+
+
 @pytest.mark.parametrize("thermostat,noise,rc_dpd", [(False, 0, None), (True, 0, None), (True, 1, None),
                                                       (True, 1, 1.0)])
 def test_pair_forces_fp64(oracle_mod, thermostat, noise, rc_dpd):
