@@ -30,3 +30,19 @@ def oracle_mod():
     from oracle import oracle
     oracle.load()
     return oracle
+
+
+REF_INPUT_CASES = ("Filled_Channel", "Fully_Flexible_Brush_Channel")
+
+
+def load_ref_input(name):
+    """One of the reference's shipped channel input sets (inputs/<name>):
+    (params, a_type, r0, v) from tests/golden/ref_inputs.npz."""
+    import json
+    from mfa_brush_amd.params import Flags, derive_tables, read_mfa_input, read_system_input
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ref_inputs.npz"))
+    fl = Flags(**json.loads(str(z[name + "/flags"])))
+    si = read_system_input(str(z[name + "/system_input"]), fl)
+    mi = read_mfa_input(str(z[name + "/mfa_input"]), fl)
+    p = derive_tables(fl, si, mi)
+    return p, z[name + "/a_type"].astype(np.int32), z[name + "/r0"], z[name + "/v"]

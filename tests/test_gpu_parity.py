@@ -78,6 +78,39 @@ def test_try_trajectory_follows_fort41(try_fixture, precision):
             assert err[k] < tol, (5 * (i + 1), k, err[k])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["Filled_Channel", "Fully_Flexible_Brush_Channel"])
+def test_ref_input_sets_vs_oracle(oracle_mod, name, precision):
+    """The reference's shipped channel configurations (inputs/<name>, with
+    the flags of their control_simulation.h): the brute-force energy, the
+    pair set (per-bead partner counts and hashes) and the pair forces with
+    the DPD terms at the same Philox noise against the oracle (FP64 1e-11 of
+    max|F|, mixed 1e-5), and 10 fused FP64 steps against the oracle's."""
+    from conftest import load_ref_input
+    p, t, r0, v = load_ref_input(name)
+    p = p.replace(skin=0.3)
+    o = _oracle(oracle_mod, p)
+    step = 11
+    with Simulation(p, precision=precision, t_idx0=step) as s:
+        s.set_state(r0, v, t)
+        assert s.check_fluid_fluid() == pytest.approx(o.check_fluid_fluid(r0, t), rel=1e-10)
+        s.fluid_fluid()
+        f = s.get_state()["force"]
+        cnt, h = s.pair_stats()
+    fo, eo = o.fluid_fluid(r0, v, t, step=step, noise=1)
+    co, ho = o.pair_stats(r0, t)
+    assert np.array_equal(cnt, co) and np.array_equal(h, ho)
+    _close(f, fo, FTOL if precision == "fp64" else 1e-5)
+    if precision == "fp64":
+        with Simulation(p, t_idx0=step) as s:
+            s.set_state(r0, v, t)
+            s.md_steps(10)
+            st = s.get_state()
+        ro, vo, *_ = o.md_steps(r0, v, np.zeros_like(r0), t, step, 10, noise=1)
+        _close(st["r0"], ro, 1e-9)
+        _close(st["v"], vo, 1e-6)
+
+
 @pytest.mark.parametrize("thermostat,noise,rc_dpd", [(False, 0, None), (True, 0, None), (True, 1, None),
                                                       (True, 1, 1.0)])
 def test_pair_forces_fp64(oracle_mod, thermostat, noise, rc_dpd):
