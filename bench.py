@@ -8,9 +8,11 @@ on the skin criterion (skin 0.4 by default, --skin).  A step is one full md_main
 integrate, cell maintenance, LJ+DPD pairs, bonds, walls, second kick.
 
 value = beads x steps / seconds of the mixed-precision path (FP32 pair
-arithmetic, FP64 integration and sums); the plain FP64 path is timed the same
-way and reported beside it.  The pair kernel's average launch duration comes
-from HIP events on the library's stream over the timed steps.
+arithmetic, FP64 integration and sums): after 100 warm-up steps, 5 timed runs
+of --steps steps each, one after the other, and the median run is the value
+(every run's rate is in "runs"); the plain FP64 path is timed the same way and
+reported beside it.  The pair kernel's average launch duration comes from HIP
+events on the library's stream over the timed steps.
 
 time_to_equilibrium: a bounded run of the learned warm start (training
 profiles from 64 small batched channels, surrogate training, then a held-out
@@ -45,16 +47,17 @@ CALIBRATION = ("oracle/calibrate.py: port/reference = 2.63 at 8 threads (33k bea
                "equivalent figure is an upper bound on the reference's own rate")
 
 
-def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None):
-    """Warm-up steps (graph replay), then the timed region: `steps` steps
-    launched in order with a pair of HIP events around each pair-kernel
-    launch on the library's stream (so the pair kernel's average launch
-    duration comes from the same steps as the wall time; two event records
-    per step cost about 2.5 % against graph replay, eight about 15 %).
-    After it, outside the timed region: steps with events around every
-    kernel group for the per-kernel breakdown, and steps replayed as graphs
-    without events.  ps and states list the channels of one (possibly
-    batched) context."""
+def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None, repeats=1):
+    """Warm-up steps (graph replay), then `repeats` timed runs of `steps`
+    steps each, one after the other, each bracketed by a synchronisation:
+    the steps are launched in order with a pair of HIP events around each
+    pair-kernel launch on the library's stream (so the pair kernel's average
+    launch duration comes from the same steps as the wall time; two event
+    records per step cost about 2.5 % against graph replay, eight about
+    15 %).  After them, outside the timed region: steps with events around
+    every kernel group for the per-kernel breakdown, and steps replayed as
+    graphs without events.  ps and states list the channels of one
+    (possibly batched) context."""
     from mfa_brush_amd.sim import Simulation
     s = Simulation(ps if len(ps) > 1 else ps[0], precision=precision)
     if choice is not None:
@@ -66,11 +69,13 @@ def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None):
         s.set_state(r0, v, t, isys=i)
     s.md_steps(warmup, graph=True)
     s.set_timing(2)
-    s.synchronize()
-    t0 = time.perf_counter()
-    s.md_steps(steps, graph=True)
-    s.synchronize()
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(repeats):
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.md_steps(steps, graph=True)
+        s.synchronize()
+        runs.append(time.perf_counter() - t0)
     pair_ms, pair_n = s.kernel_time("pair")
     n_break = max(8, min(steps, 40))
     s.set_timing(1)
@@ -88,8 +93,9 @@ def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None):
     fl = s.flags()
     cap = s.neighbor_list()
     s.close()
-    return dict(seconds=dt, seconds_plain=dt_plain, pair_ms=pair_ms, pair_n=pair_n, npairs=npairs, flags=fl,
-                list_cap=cap, kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
+    return dict(seconds=sorted(runs)[len(runs) // 2], runs=runs, seconds_plain=dt_plain, pair_ms=pair_ms,
+                pair_n=pair_n, npairs=npairs, flags=fl, list_cap=cap,
+                kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
 
 
 def build_workload(name, skin, n_rep=256):
@@ -177,8 +183,10 @@ def cpu_baseline(ps, states, n_steps, max_channels=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200, help="steps per timed run")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed runs of --steps steps, one after the other; value is the median run")
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="steps of the CPU baseline sample (10-20 s of CPU work at 1M beads on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -216,12 +224,16 @@ def main():
         args.skin = tuned["skin"]
 
     nlist = 0 if args.no_list else -1
-    mixed = run_gpu("mixed", args.steps, args.warmup, ps, states, nlist, tuned)
-    print("mixed: %.3f s for %d steps" % (mixed["seconds"], args.steps), file=sys.stderr, flush=True)
+    if args.repeats < 1:
+        ap.error("--repeats must be at least 1")
+    mixed = run_gpu("mixed", args.steps, args.warmup, ps, states, nlist, tuned, args.repeats)
+    print("mixed: median %.4f s for %d steps (%d runs)" % (mixed["seconds"], args.steps, args.repeats),
+          file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_gpu("fp64", args.steps, args.warmup, ps, states, nlist)
-        print("fp64: %.3f s for %d steps" % (fp64["seconds"], args.steps), file=sys.stderr, flush=True)
+        fp64 = run_gpu("fp64", args.steps, args.warmup, ps, states, nlist, None, args.repeats)
+        print("fp64: median %.4f s for %d steps (%d runs)" % (fp64["seconds"], args.steps, args.repeats),
+              file=sys.stderr, flush=True)
 
     value = n * args.steps / mixed["seconds"]
     pair_avg_s = mixed["pair_ms"] / max(mixed["pair_n"], 1) / 1e3
@@ -233,6 +245,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "repeats": args.repeats,
+        "runs": [n * args.steps / t for t in mixed["runs"]],
         "ms_per_step": 1e3 * mixed["seconds"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -254,7 +268,8 @@ def main():
     if fp64 is not None:
         v64 = n * args.steps / fp64["seconds"]
         p64 = fp64["pair_ms"] / max(fp64["pair_n"], 1) / 1e3
-        out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+        out["fp64"] = {"value": v64, "runs": [n * args.steps / t for t in fp64["runs"]],
+                       "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
                        "graph_no_event_ms_per_step": 1e3 * fp64["seconds_plain"] / args.steps,
                        "pair_avg_us": p64 * 1e6,
                        "pair_frac_fp64_peak": FLOPS_PER_PAIR * fp64["npairs"] / p64 / PEAK_FP64,
