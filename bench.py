@@ -16,9 +16,9 @@ events on the library's stream over the timed steps.
 
 time_to_equilibrium: a bounded run of the learned warm start (training
 profiles from 64 small batched channels, surrogate training, then a held-out
-12.8k-bead channel run for 100 blocks of 1000 steps from the plain and from
-the warm start side by side), blocks to the 10 / 5 / 2 % profile bands for
-both starts.
+25k-bead channel run for 100 blocks of 1000 steps from the lattice start, the
+reference's generator start and the warm start side by side), blocks to the
+10 / 5 / 2 % profile bands for the three starts.
 """
 import argparse
 import json

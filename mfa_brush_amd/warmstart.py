@@ -303,7 +303,7 @@ def bounded_tte(n_train: int = 64, equil: int = 20000, avg: int = 20000, n_block
     t_train = time.perf_counter()
     mlp, hist = train(specs, profs, steps=mlp_steps, batch=batch)
     t_train = time.perf_counter() - t_train
-    held = held or Spec(0.25, 16, 1.0, side=20.0, seed=907)
+    held = held or Spec(0.25, 16, 1.0, side=28.0, seed=907)
     starts = [build(held, None), build_generated(held), build(held, mlp)]
     runs, n_part, secs, fl = block_profiles_starts(starts, n_blocks, block=block)
     late = [r[n_blocks // 2:].mean(0) for r in runs]
@@ -424,7 +424,7 @@ def main_tte(argv):
     mlp_steps batch]: one bounded_tte run, written as JSON."""
     out_path = argv[0]
     vals = [float(x) for x in argv[1:]]
-    d = [64, 20000, 20000, 100, 20.0, 0.25, 16, 1.0, 6000, 512]
+    d = [64, 20000, 20000, 100, 28.0, 0.25, 16, 1.0, 6000, 512]
     vals = vals + d[len(vals):]
     held = Spec(vals[5], int(vals[6]), vals[7], side=vals[4], seed=907)
     r = bounded_tte(n_train=int(vals[0]), equil=int(vals[1]), avg=int(vals[2]), n_blocks=int(vals[3]), held=held,
