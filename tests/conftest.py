@@ -32,7 +32,7 @@ def oracle_mod():
     return oracle
 
 
-REF_INPUT_CASES = ("Filled_Channel", "Fully_Flexible_Brush_Channel")
+REF_INPUT_CASES = ("Filled_Channel", "Fully_Flexible_Brush_Channel", "Clog")
 
 
 def load_ref_input(name):

@@ -8,9 +8,9 @@ data the reference ships:
   system_input / mfa_input text, the brute-force energy the reference
   printed for it (try/log:152) and the first rows of fort.41;
 * inputs/Filled_Channel and inputs/Fully_Flexible_Brush_Channel (the
-  channel set-ups, SYSTEM 0): conf_old positions, velocities and types,
-  system_input / mfa_input text, and the hot-path flags of their
-  control_simulation.h.
+  channel set-ups, SYSTEM 0) and inputs/Clog (a droplet in the channel,
+  SYSTEM 1): conf_old positions, velocities and types, system_input /
+  mfa_input text, and the hot-path flags of their control_simulation.h.
 """
 import os
 import sys
@@ -28,12 +28,20 @@ TRY_FLAGS = Flags(solvent=3, brush_in_good_sv=True, dpd_cut_off=2.24, bending=Tr
 
 
 # hot-path flags of the input sets' control_simulation.h (ORIENTATION,
-# which Filled_Channel also defines, is outside the path)
+# which Filled_Channel and Clog also define, is outside the path).  Clog is
+# SYSTEM 1, which the force and integration routines treat as SYSTEM 0 (its
+# droplet reshaping, verlet_positions.f90:270-274, runs only while i_time <=
+# n_relax, and its start time is past that); its THERMOSTAT 1 (Langevin)
+# adds no thermostat force in the reference's current fluid_fluid.f90, whose
+# lgv_forces call is commented out (:455-458), so it runs with the
+# thermostat off.
 INPUT_FLAGS = {
     "Filled_Channel": dict(solvent=3, brush_in_good_sv=True, bending=True, thermostat=True, dpd_cut_off=2.24,
                            force_switch_on=True, poiseuille=False),
     "Fully_Flexible_Brush_Channel": dict(solvent=2, brush_in_good_sv=False, bending=False, thermostat=True,
                                          dpd_cut_off=2.24, force_switch_on=True, poiseuille=False),
+    "Clog": dict(solvent=3, brush_in_good_sv=True, bending=True, thermostat=False, dpd_cut_off=2.24,
+                 force_switch_on=True, poiseuille=True),
 }
 
 

@@ -15,6 +15,7 @@ import pytest
 from mfa_brush_amd import configs
 from mfa_brush_amd.params import Flags
 from mfa_brush_amd.sim import Simulation
+from conftest import REF_INPUT_CASES
 
 pytestmark = pytest.mark.gpu
 
@@ -79,7 +80,7 @@ def test_try_trajectory_follows_fort41(try_fixture, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("name", ["Filled_Channel", "Fully_Flexible_Brush_Channel"])
+@pytest.mark.parametrize("name", REF_INPUT_CASES)
 def test_ref_input_sets_vs_oracle(oracle_mod, name, precision):
     """The reference's shipped channel configurations (inputs/<name>, with
     the flags of their control_simulation.h): the brute-force energy, the
