@@ -126,8 +126,13 @@ int  mfa_hip_get_time(mfa_ctx *ctx, int64_t *t_idx);
 
 /* ---- step routines (drop-in for md_main.f90:85-185) -------------------- */
 int  mfa_hip_verlet_positions(mfa_ctx *ctx);       /* verlet_positions.f90:150-209 + cell relink/rebuild */
-int  mfa_hip_check_skin(mfa_ctx *ctx, int32_t *n_rebuilds); /* check_skin.f90:4-28: the skin test runs inside
-                                                      verlet_positions; this reports the rebuild count */
+int  mfa_hip_check_skin(mfa_ctx *ctx, int32_t *f_skin); /* check_skin.f90:4-28: f_skin of the last
+                                                      verlet_positions, 1 when its skin test fired (some bead
+                                                      more than skin/2 from its position at the last rebuild),
+                                                      else 0.  The skin test and the rebuild it calls for
+                                                      (binning.f90 / my_binning.f90, md_main.f90:97-102) run
+                                                      inside verlet_positions, so the rebuild is done by then;
+                                                      the rebuild count is in mfa_flags */
 int  mfa_hip_fluid_fluid(mfa_ctx *ctx);            /* fluid_fluid.f90:1-512 + dpd_forces.f90; also
                                                       performs md_main.f90:107 force(:,:)=0 */
 int  mfa_hip_fluid_wall(mfa_ctx *ctx, int32_t inter_type);   /* fluid_wall.f90:77-123 (inter_type 2) */

@@ -72,6 +72,7 @@ struct mfa_ctx {
     int *d_tiles = nullptr;        // persistent mixed pair kernel: work counter, ticket counter
     int pair_grid = 0;             // 0: one block per tile; > 0: persistent launch of this many blocks
     int *d_rebuild_count = nullptr, *d_permute_done = nullptr;
+    int *d_skin_fired = nullptr;   // whether the last verlet_positions' skin test fired (f_skin)
     double *d_obs = nullptr;       // block partials of the chain observables (2 x channel blocks x 8)
     double *d_stage = nullptr;
     int *d_istage = nullptr;
@@ -196,7 +197,7 @@ static void free_all(mfa_ctx *c) {
     F(c->B.nl); F(c->B.nl_n);
     F(c->d_sys); F(c->d_blk_sys); F(c->d_cell_sys); F(c->d_cell_count); F(c->d_pre); F(c->d_bsum);
     F(c->d_crange); F(c->d_bin_cell); F(c->d_bin_rank); F(c->d_perm); F(c->d_key_tmp); F(c->d_bin_key); F(c->d_partial);
-    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_permute_done); F(c->d_obs); F(c->d_stage); F(c->d_istage);
+    F(c->d_energies); F(c->d_fheads); F(c->d_rebuild_count); F(c->d_skin_fired); F(c->d_permute_done); F(c->d_obs); F(c->d_stage); F(c->d_istage);
     F(c->d_pcnt); F(c->d_pvs); F(c->d_pv2s); F(c->d_pdens); F(c->d_pvp); F(c->d_pvp2); F(c->d_dfd); F(c->d_tiles);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -721,7 +722,7 @@ int mfa_hip_create(const mfa_system_params *sys, int32_t n_sys, int32_t precisio
     ALLOC(c->d_bin_cell, c->S_tot); ALLOC(c->d_bin_rank, c->S_tot); ALLOC(c->d_perm, c->S_tot);
     ALLOC(c->d_key_tmp, c->S_tot); ALLOC(c->d_bin_key, c->S_tot);
     ALLOC(c->d_partial, (size_t)c->n_blocks * NPART); ALLOC(c->d_energies, (size_t)n_sys * NPART);
-    ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1); ALLOC(c->d_permute_done, 1);
+    ALLOC(c->d_fheads, 3 * (size_t)std::max(1, c->H_tot)); ALLOC(c->d_rebuild_count, 1); ALLOC(c->d_skin_fired, 1); ALLOC(c->d_permute_done, 1);
     {
         int mb = 1;
         for (const SysDev &d : c->hs) mb = std::max(mb, (d.n_part + BLK - 1) / BLK + 1);
@@ -795,7 +796,7 @@ int mfa_hip_set_state(mfa_ctx *c, int32_t isys, const double *r0, const double *
 static int ensure_cells(mfa_ctx *c) {
     // A rebuild is pending after set_state; run it (it is a no-op otherwise).
     enqueue_rebuild(c, 1);
-    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
+    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys, (int *)nullptr);
     return 0;
 }
 
@@ -840,7 +841,7 @@ int mfa_hip_verlet_positions(mfa_ctx *c) {
     ensure_cells(c);
     enqueue_vv1(c);
     enqueue_rebuild(c, 1);
-    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys);
+    hipLaunchKernelGGL(k_clear_rebuild, dim3(1), dim3(BLK), 0, c->st, c->B, c->n_sys, c->d_skin_fired);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -848,10 +849,12 @@ int mfa_hip_verlet_positions(mfa_ctx *c) {
 int mfa_hip_check_skin(mfa_ctx *c, int32_t *f_skin) {
     if (int rc = check_ctx(c)) return rc;
     if (!f_skin) return fail(MFA_E_ARG, "check_skin: null");
-    // verlet_positions already rebuilt the cells when the skin test fired;
-    // report how many rebuilds happened so far.
+    // f_skin of the last verlet_positions: 1 when its skin test fired (some
+    // bead more than skin/2 from its position at the last rebuild), else 0.
+    // verlet_positions has already rebuilt the cells and lists by then, which
+    // is the binning the reference calls next (md_main.f90:97-102).
     int v = 0;
-    if (int rc = to_host(c, &v, c->d_rebuild_count, sizeof(int), "rebuild count")) return rc;
+    if (int rc = to_host(c, &v, c->d_skin_fired, sizeof(int), "skin test flag")) return rc;
     *f_skin = v;
     return 0;
 }

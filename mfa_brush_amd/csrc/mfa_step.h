@@ -541,9 +541,15 @@ __global__ __launch_bounds__(BLK) void k_mark_rebuild(Bufs B, int s) {
 }
 
 // Clear the rebuild requests of all n_sys channels.
-__global__ __launch_bounds__(BLK) void k_clear_rebuild(Bufs B, int n_sys) {
+// Clears the rebuild gates once the gated rebuild has run.  fired (may be
+// null): set to whether the gates were up, i.e. whether the skin test of the
+// step fired (f_skin after check_skin.f90:11-24).
+__global__ __launch_bounds__(BLK) void k_clear_rebuild(Bufs B, int n_sys, int *fired) {
     for (int s = threadIdx.x; s < n_sys; s += BLK) B.chan_rb[s] = 0;
-    if (threadIdx.x == 0) *B.rebuild = 0;
+    if (threadIdx.x == 0) {
+        if (fired) *fired = *B.rebuild != 0 ? 1 : 0;
+        *B.rebuild = 0;
+    }
 }
 
 __global__ __launch_bounds__(BLK) void k_bin(const SysDev *__restrict__ sys, const int *__restrict__ blk_sys, Bufs B,

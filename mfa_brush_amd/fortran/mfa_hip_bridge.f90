@@ -70,6 +70,11 @@ module mfa_hip_iface
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function
+        integer(c_int) function mfa_hip_check_skin(ctx, f_skin) bind(C, name="mfa_hip_check_skin")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: f_skin
+        end function
         integer(c_int) function mfa_hip_fluid_fluid(ctx) bind(C, name="mfa_hip_fluid_fluid")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
@@ -212,7 +217,11 @@ contains
 #else
         p%poiseuille = 0
 #endif
+#ifdef L_BOX
         p%l_box = L_BOX
+#else
+        p%l_box = 1                          ! headers without L_BOX (Verlet-list builds): cells of one cutoff
+#endif
         p%bend_form = 0
 #ifdef DPD_VV
         p%dpd_vv = 1
@@ -235,7 +244,11 @@ contains
 #else
         p%r_cut_dpd_2 = 0.0d0
 #endif
+#if BIN_TYPE == 0 || BIN_TYPE == 1
+        p%skin = skin                        ! init_params.f90:823-835; rebuild past skin/2 (check_skin.f90:21)
+#else
         p%skin = 0.0d0                       ! BIN_TYPE 2: cells rebuilt every step
+#endif
         p%sig = sig; p%friction = friction(1)
         p%r_2_min_init = r_2_min_init; p%r_2_min_time = r_2_min_time
         p%sigma_wall = sigma_wall(1:4); p%a_wall = a_wall(1:4); p%a_w = a_w
@@ -281,6 +294,33 @@ subroutine verlet_positions()                     ! verlet_positions.f90:1
     call mfa_chk(mfa_hip_set_time(mfa_ctx, int(s_time + i_time - 1, c_int64_t)), "mfa_hip_set_time")
     call mfa_chk(mfa_hip_verlet_positions(mfa_ctx), "verlet_positions")
 end subroutine verlet_positions
+
+! BIN_TYPE 0 / 1 drivers call check_skin after verlet_positions and, when
+! f_skin is 1, binning or my_binning (md_main.f90:92-102; also once at the
+! start, init_config.f90:255-257, conf_default.f90:202-204).  The device runs
+! the skin test and the rebuild inside verlet_positions and keeps its own
+! lists, so check_skin reports the device's f_skin and the two binning
+! routines only reset it (binning.f90:125, my_binning.f90:130); the host
+! lists they filled (ff_list, r0_old, mic_old) were read only by the replaced
+! fluid_fluid, new_dpd_fd and check_skin.
+subroutine check_skin()                           ! check_skin.f90:4
+    use commons, only: f_skin
+    use mfa_hip_state
+    integer(c_int32_t) :: fs
+    call mfa_bridge_ensure()
+    call mfa_chk(mfa_hip_check_skin(mfa_ctx, fs), "check_skin")
+    f_skin = fs
+end subroutine check_skin
+
+subroutine binning()                              ! binning.f90:2
+    use commons, only: f_skin
+    f_skin = 0
+end subroutine binning
+
+subroutine my_binning()                           ! my_binning.f90:5
+    use commons, only: f_skin
+    f_skin = 0
+end subroutine my_binning
 
 subroutine fluid_fluid()                          ! fluid_fluid.f90:1 (includes md_main.f90:107 force=0)
     use mfa_hip_state

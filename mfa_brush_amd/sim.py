@@ -117,6 +117,8 @@ class Simulation:
         self._c(self.lib.mfa_hip_verlet_positions(self.h), "verlet_positions")
 
     def check_skin(self) -> int:
+        """f_skin after the last verlet_positions (check_skin.f90:4-28): 1
+        when its skin test fired, else 0; the rebuild it calls for is done."""
         f = C.c_int32()
         self._c(self.lib.mfa_hip_check_skin(self.h, C.byref(f)), "check_skin")
         return f.value

@@ -291,6 +291,41 @@ def test_neighbor_list_equals_cell_scan(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_check_skin_reports_f_skin(precision):
+    """check_skin after verlet_positions is f_skin of check_skin.f90:11-24:
+    1 on the steps whose skin test fired (a bead more than skin/2 from its
+    position at the last rebuild), else 0, and each such step rebuilt the
+    cells once.  One solvent bead drifts 0.15 sigma a step while every other
+    bead rests (no forces are evaluated, so nothing accelerates), so with
+    skin 0.4 the test fires every second step."""
+    p, t, r0, v = configs.channel_c1(thermostat=False)
+    p = p.replace(skin=0.4)
+    k = p.n_part - 1                                  # a solvent bead
+    r0 = r0.copy()
+    r0[k, 0] = 3.0                                    # far from the periodic wrap in x
+    v = np.zeros_like(v)
+    v[k, 0] = 0.15 / p.dt
+    expect, d = [], 0.0
+    for _ in range(6):
+        d += 0.15
+        fired = d * d > (0.5 * p.skin) ** 2
+        expect.append(int(fired))
+        d = 0.0 if fired else d
+    assert expect == [0, 1, 0, 1, 0, 1]
+    got, rebuilds = [], []
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t)
+        for _ in range(6):
+            s.verlet_positions()
+            got.append(s.check_skin())
+            rebuilds.append(s.flags()["rebuilds"])
+        assert abs(s.get_state()["r0"][k, 0] - (3.0 + 6 * 0.15)) < 1e-12
+    assert got == expect
+    # the first call also ran the rebuild pending after set_state
+    assert list(np.diff(rebuilds)) == expect[1:]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("rc_dpd", [1.0, None])
 def test_dpd_queue_depths_agree(oracle_mod, precision, rc_dpd):
     """The DPD part of a pair runs from the per-lane queue or, once the queue
