@@ -10,11 +10,17 @@
  * -ffp-contract=off, so the interacting-pair decision (r^2 < range_2) is the
  * same IEEE computation as the reference's (no FMA on the decision path).
  *
- * Parity pins (tests/test_oracle.py): try/conf_old brute-force V_ff golden
- * value, the real Fortran reference built into oracle/_ref (thermostat-off
- * and dissipative-only forces, bonds, walls, bending, integrator), finite
+ * Parity pins: try/conf_old brute-force V_ff golden value, finite
  * differences of every potential, Newton-3 / momentum invariants, the
- * Random123 Philox known-answer vectors, brute force vs cell list.
+ * Random123 Philox known-answer vectors, brute force vs cell list
+ * (tests/test_oracle.py); the real Fortran reference built into oracle/_ref
+ * (thermostat-off and dissipative-only forces, bonds, walls, bending,
+ * integrator; tests/test_oracle_vs_reference.py and the golden vectors it
+ * wrote); closed forms of the DPD_CUT_OFF branch, of the cosine bending and
+ * of the pair and bond virial, the bending_melt range, the virial as the
+ * strain derivative of the energy, and the reference's own try/fort.41
+ * energy record over 30 steps (tests/test_oracle_pins.py); the reference's
+ * shipped input sets, cell list vs brute force (tests/test_ref_inputs_host.py).
  */
 #include "mfa_oracle.h"
 #include <math.h>
