@@ -64,6 +64,15 @@ constexpr int DPD_UNROLL = 2;     // deferred pairs whose partner loads are issu
 __device__ __forceinline__ size_t nl_index(int k, int q, size_t stride) {
     return ((size_t)(q >> 2) * stride + (size_t)k) * 4 + (size_t)(q & 3);
 }
+// One 16-byte group of list entries, loaded with the non-temporal (streaming)
+// hint: a launch reads the list once, while the partner coordinates it
+// points to are gathered tens of times, so the list should not displace
+// them from the caches (C2: pair step 173 -> 165 us, whole step -4 %).
+__device__ __forceinline__ int4 nl_group(const int *__restrict__ nl, size_t idx) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i g = __builtin_nontemporal_load((const v4i *)(nl + idx));
+    return make_int4(g.x, g.y, g.z, g.w);
+}
 // Slot i of a per-slot array through an unsigned 32-bit byte offset, so the
 // load takes the uniform base in scalar registers and one 32-bit lane offset
 // (no 64-bit address add per gather).  Lists hold slots below 2^23 (the host
@@ -505,7 +514,7 @@ __global__ __launch_bounds__(BLK) void k_pair64(const SysDev *__restrict__ sys, 
             for (int q0 = 0; q0 < nn; q0 += NL_UNROLL) {
                 unsigned mm[NL_UNROLL];
                 double4 pj[NL_UNROLL];
-                const int4 e4 = *(const int4 *)(B.nl + nl_index(k, q0, (size_t)B.n_slots));
+                const int4 e4 = nl_group(B.nl, nl_index(k, q0, (size_t)B.n_slots));
                 // unsigned slots: one compare is the range check (as in k_pair32)
                 mm[0] = (unsigned)e4.x >> NL_SLOT_SHIFT; mm[1] = (unsigned)e4.y >> NL_SLOT_SHIFT;
                 mm[2] = (unsigned)e4.z >> NL_SLOT_SHIFT; mm[3] = (unsigned)e4.w >> NL_SLOT_SHIFT;
@@ -817,7 +826,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
             // row's last group, is loaded and not used).
             const int trow8p1 = trow8 + 1;
             const int q_last = B.nl_cap - NL_UNROLL;
-            int4 e4 = *(const int4 *)(B.nl + nl_index(k, 0, (size_t)B.n_slots));
+            int4 e4 = nl_group(B.nl, nl_index(k, 0, (size_t)B.n_slots));
             auto group = [&](int q0, auto tail) {
                 const int ent[NL_UNROLL] = {e4.x, e4.y, e4.z, e4.w};
                 unsigned mm[NL_UNROLL];
@@ -828,7 +837,7 @@ __device__ __forceinline__ void pair32_tile(const int tile, const SysDev *__rest
                     pj[u] = *(const float3 *)((const char *)posf + mm[u] * (unsigned)sizeof(float4));
                 }
                 if (!decltype(tail)::value)
-                    e4 = *(const int4 *)(B.nl + nl_index(k, min(q0 + NL_UNROLL, q_last), (size_t)B.n_slots));
+                    e4 = nl_group(B.nl, nl_index(k, min(q0 + NL_UNROLL, q_last), (size_t)B.n_slots));
 #pragma unroll
                 for (int u = 0; u < NL_UNROLL; ++u) {
                     if (!decltype(tail)::value || q0 + u < nn)
