@@ -92,9 +92,13 @@ def run_gpu(precision, steps, warmup, ps, states, nlist=-1, choice=None, repeats
     npairs = sum(s.pair_count(i) for i in range(len(ps)))
     fl = s.flags()
     cap = s.neighbor_list()
+    lists = None
+    if cap > 0:                     # the lists of the last rebuild, summed over the channels
+        per = [s.list_stats(i) for i in range(len(ps))]
+        lists = {k: sum(x[k] for x in per) for k in per[0]}
     s.close()
     return dict(seconds=sorted(runs)[len(runs) // 2], runs=runs, seconds_plain=dt_plain, pair_ms=pair_ms,
-                pair_n=pair_n, npairs=npairs, flags=fl, list_cap=cap,
+                pair_n=pair_n, npairs=npairs, flags=fl, list_cap=cap, lists=lists,
                 kernels={k: {"total_ms": a, "launches": b} for k, (a, b) in tot.items()})
 
 
@@ -141,6 +145,19 @@ KERNEL_LABELS = {"vv1": "vv1_once_per_call", "rebuild": "rebuild", "pair": "pair
 
 def kernels_ms(kernels):
     return {KERNEL_LABELS.get(k, k): d["total_ms"] / max(d["launches"], 1) for k, d in kernels.items()}
+
+
+def neighbor_list(res, ps):
+    """Verlet-list figures of a run: capacity, overflows, and from the
+    lists of the last rebuild the entries per bead and the entries a wave of
+    64 slots walks (its longest lane's list, in groups of four)."""
+    out = {"capacity": res["list_cap"], "overflowed_beads": res["flags"]["list_overflow"]}
+    if res["lists"]:
+        n = sum(p.n_part for p in ps)
+        waves = sum((p.n_part + 63) // 64 for p in ps)
+        out["entries_per_bead"] = res["lists"]["entries"] / n
+        out["walked_per_wave"] = res["lists"]["wave_walk"] / waves
+    return out
 
 
 def calib_copy(mib):
@@ -263,7 +280,7 @@ def main():
         "kernels_ms": kernels_ms(mixed["kernels"]),
         "graph_no_event_ms_per_step": 1e3 * mixed["seconds_plain"] / args.steps,
         "rebuilds": mixed["flags"]["rebuilds"],
-        "neighbor_list": {"capacity": mixed["list_cap"], "overflowed_beads": mixed["flags"]["list_overflow"]},
+        "neighbor_list": neighbor_list(mixed, ps),
     }
     if fp64 is not None:
         v64 = n * args.steps / fp64["seconds"]

@@ -242,6 +242,14 @@ int  mfa_hip_get_pair_count(mfa_ctx *ctx, int32_t isys, int64_t *n_pairs);  /* i
  * out[0..5] on the host. */
 int  mfa_hip_chain_observables(mfa_ctx *ctx, int32_t isys, double *out);
 
+/* Verlet-list statistics of channel isys (no reference counterpart; a
+ * diagnostic of the pair kernels' work, outside the step): out[0] the list
+ * entries of its beads, out[1] the entries its waves walk (a wave of 64
+ * slots walks the longest list of its lanes, in groups of four), out[2] its
+ * beads whose list overflowed the capacity (they scan their cells).  Lists
+ * are those of the last rebuild; fails when the list is off (capacity 0). */
+int  mfa_hip_list_stats(mfa_ctx *ctx, int32_t isys, int64_t *out);
+
 /* DPD pairs each bead defers (0..8, default 8).  The pair kernels evaluate
  * the LJ part of every pair while walking the partners and queue the DPD part
  * (Philox normal, random and dissipative forces) of up to `depth` pairs per

@@ -94,6 +94,7 @@ SIGNATURES = {
     "mfa_hip_set_dpd_queue": [_vp, _i32],
     "mfa_hip_set_pair_grid": [_vp, _i32],
     "mfa_hip_chain_observables": [_vp, _i32, _vp],
+    "mfa_hip_list_stats": [_vp, _i32, _vp],
     "mfa_hip_get_pair_count": [_vp, _i32, _vp],
     "mfa_hip_last_error": [],
     "mfa_hip_version": [],

@@ -1017,6 +1017,23 @@ int mfa_hip_get_f_on_heads(mfa_ctx *c, int32_t isys, double *f) {
     return to_host(c, f, c->d_fheads + 3 * (size_t)d.head_base, 3 * (size_t)d.n_chain * sizeof(double), "f_on_heads");
 }
 
+int mfa_hip_list_stats(mfa_ctx *c, int32_t isys, int64_t *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (isys < 0 || isys >= c->n_sys || !out) return fail(MFA_E_ARG, "list_stats: bad argument");
+    if (c->B.nl_cap <= 0) return fail(MFA_E_STATE, "list_stats: no Verlet list (capacity 0)");
+    ensure_cells(c);                  // the rebuild pending after set_state builds the lists
+    const SysDev &d = c->hs[isys];
+    const int gm = std::max(1, (d.n_part + BLK - 1) / BLK);   // 3 gm partials fit in d_obs (16 per block)
+    LAUNCH(k_list_stats, gm, c->st, c->d_sys, isys, c->B, c->d_obs);
+    std::vector<double> h(3 * (size_t)gm);
+    if (int rc = to_host(c, h.data(), c->d_obs, h.size() * sizeof(double), "list statistics")) return rc;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < gm; ++b)
+        for (int q = 0; q < 3; ++q) t[q] += h[3 * (size_t)b + q];
+    for (int q = 0; q < 3; ++q) out[q] = (int64_t)t[q];
+    return 0;
+}
+
 int mfa_hip_chain_observables(mfa_ctx *c, int32_t isys, double *out) {
     if (int rc = check_ctx(c)) return rc;
     if (isys < 0 || isys >= c->n_sys || !out) return fail(MFA_E_ARG, "chain_observables: bad argument");

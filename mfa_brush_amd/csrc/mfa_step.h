@@ -964,6 +964,36 @@ __global__ __launch_bounds__(BLK) void k_momentum(const SysDev *__restrict__ sys
     }
 }
 
+// Verlet-list statistics of channel s (a diagnostic, outside the step): per
+// block of slots, part[3 b + 0] the list entries of its beads, part[3 b + 1]
+// the entries its waves walk (a wave walks the longest list of its lanes, in
+// groups of four) and part[3 b + 2] its beads whose list overflowed (they scan
+// their cells instead).
+__global__ __launch_bounds__(BLK) void k_list_stats(const SysDev *__restrict__ sys, int s, Bufs B,
+                                                   double *__restrict__ part) {
+    const SysDev &P = sys[s];
+    __shared__ int nn[BLK];
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    nn[threadIdx.x] = i < P.n_part ? B.nl_n[P.slot_base + i] : -1;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double ent = 0.0, walk = 0.0, over = 0.0;
+    for (int w = 0; w < BLK; w += 64) {
+        int mx = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int n = nn[w + l];
+            if (n < 0) continue;
+            if (n > B.nl_cap) { over += 1.0; continue; }
+            ent += (double)n;
+            mx = max(mx, n);
+        }
+        walk += (double)((mx + 3) / 4 * 4);
+    }
+    part[3 * (size_t)blockIdx.x] = ent;
+    part[3 * (size_t)blockIdx.x + 1] = walk;
+    part[3 * (size_t)blockIdx.x + 2] = over;
+}
+
 __global__ __launch_bounds__(BLK) void k_prof_fold(int n_cnt, int n_v, unsigned int *__restrict__ cnt,
                                                   unsigned long long *__restrict__ vs,
                                                   unsigned long long *__restrict__ v2s, double *__restrict__ dens,

@@ -208,6 +208,14 @@ class Simulation:
             aniso = np.array([(g[2] - 0.5 * (g[0] + g[1])) / (g[0] + g[1] + g[2]) for g in (gb, gm)])
         return dict(rg2_brush=gb, rg2_melt=gm, anisotropy=aniso, momentum=out[6:9])
 
+    def list_stats(self, isys: int = 0) -> dict:
+        """Verlet-list statistics of the last rebuild: list entries, the
+        entries the pair kernel's waves walk (each wave the longest list of
+        its 64 lanes, in groups of four) and overflowed beads."""
+        out = np.zeros(3, np.int64)
+        self._c(self.lib.mfa_hip_list_stats(self.h, isys, _ptr(out)), "list_stats")
+        return dict(entries=int(out[0]), wave_walk=int(out[1]), overflowed=int(out[2]))
+
     def flags(self) -> dict:
         f = _lib.Flags()
         self._c(self.lib.mfa_hip_get_flags(self.h, C.byref(f)), "get_flags")

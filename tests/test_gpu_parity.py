@@ -325,6 +325,50 @@ def test_check_skin_reports_f_skin(precision):
     assert list(np.diff(rebuilds)) == expect[1:]
 
 
+def _list_count(p, t, r0, rel):
+    """Host brute force: pairs (i, j != i) closer than the type pair's list
+    radius (interaction range, or the DPD range when the DPD thermostat acts
+    beyond it, plus the skin), the squared radius scaled by 1 + rel."""
+    r = np.sqrt(np.maximum(np.asarray(p.range_2, float), 0.0)).reshape(4, 4)
+    if p.flags.thermostat and p.flags.dpd_cut_off:
+        r = np.maximum(r, math.sqrt(p.r_cut_dpd_2))
+    rl2 = np.where(r > 0, (r + p.skin) ** 2, -1.0) * (1.0 + rel)
+    L = np.asarray(p.boundary, float)
+    ti = np.asarray(t) - 1
+    n = 0
+    for a in range(0, len(r0), 512):
+        d = r0[a:a + 512, None, :] - r0[None, :, :]
+        d[..., :2] -= L[:2] * np.rint(d[..., :2] / L[:2])
+        d2 = (d * d).sum(-1)
+        ok = d2 < rl2[ti[a:a + 512, None], ti[None, :]]
+        ok[np.arange(ok.shape[0]), a + np.arange(ok.shape[0])] = False
+        n += int(ok.sum())
+    return n
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_list_stats_vs_host_count(precision):
+    """The Verlet list of every bead holds its partners within the type
+    pair's list radius: the entry total lies between host brute-force counts
+    on the build's positions at the radius and at the radius the build
+    widens it to (FP32: 2^-12 of the squared radius, FP64: 2^-40), the waves
+    walk at least 1/64 of the entries and at most the entries plus three per
+    wave, and no list overflows (C1)."""
+    p, t, r0, v = _perturbed_c1(thermostat=True, rc_dpd=1.0)
+    p = p.replace(skin=0.3)
+    with Simulation(p, precision=precision) as s:
+        s.set_state(r0, v, t)
+        st = s.list_stats()
+    widen = 2.0 ** -12 if precision == "mixed" else 2.0 ** -40
+    lo = _list_count(p, t, r0, -1e-12 if precision == "fp64" else -1e-5)
+    hi = _list_count(p, t, r0, widen + (1e-12 if precision == "fp64" else 1e-5))
+    print(precision, st, lo, hi)
+    assert lo <= st["entries"] <= hi
+    n_waves = (len(t) + 63) // 64
+    assert st["entries"] <= 64 * st["wave_walk"] and st["wave_walk"] <= st["entries"] + 3 * n_waves
+    assert st["overflowed"] == 0
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("rc_dpd", [1.0, None])
 def test_dpd_queue_depths_agree(oracle_mod, precision, rc_dpd):
