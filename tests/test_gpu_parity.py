@@ -367,6 +367,11 @@ def test_list_stats_vs_host_count(precision):
     n_waves = (len(t) + 63) // 64
     assert st["entries"] <= 64 * st["wave_walk"] and st["wave_walk"] <= st["entries"] + 3 * n_waves
     assert st["overflowed"] == 0
+    with Simulation(p, precision=precision) as s:        # no list: the statistics are refused
+        s.set_neighbor_list(0)
+        s.set_state(r0, v, t)
+        with pytest.raises(RuntimeError, match="no Verlet list"):
+            s.list_stats()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
