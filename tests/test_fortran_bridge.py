@@ -30,7 +30,7 @@ UNITS = ["md_main", "init_system", "gen_droplet", "gen_brush", "gen_wall", "make
          "SUN", "R250", "constant_force", "messages", "dpd_forces", "ewald_real", "ewald_k", "dipolar_correction",
          "lgv_forces", "my_binning", "viscosity", "bending", "bending_melt", "orientation", "magnetic_force",
          "cell_list"]
-# the routines the bridge takes over
+# the reference units the bridge takes over (mfa_brush_amd/fortran/mfa_hip_bridge.f90 defines them)
 REPLACED = ["verlet_positions", "verlet_velocities", "fluid_fluid", "dpd_forces", "wall_wall", "fluid_wall",
             "intra_molec", "bending", "bending_melt", "constant_force", "new_dpd_fd", "check_skin", "binning",
             "my_binning"]
