@@ -721,7 +721,7 @@ __global__ __launch_bounds__(BLK) void k_permute(const SysDev *__restrict__ sys,
         NXT(B.vel)[m] = v;
         NXT(B.acc)[m] = CUR(B.acc)[old];
         if (keep_frc) NXT(B.frc)[m] = CUR(B.frc)[old];
-        NXT(B.posb)[m] = r;                            // skin reference of this build
+        NXT(B.posb)[m] = r;                            // skin reference of this rebuild
         const int cg = bin_cell[old];
         NXT(B.cell_of)[m] = cg;
         B.where[worig(r.w)] = m;

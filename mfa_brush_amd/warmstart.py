@@ -7,8 +7,8 @@ that the batched GPU simulator produces for a set of channels, and its
 prediction for a new channel seeds the initial configuration
 (configs.channel_general: monomer heights from the predicted brush density,
 solvent sites weighted by the predicted solvent density).  The plain lattice
-start (straight chains) remains the fallback, and plain MD remains the judge:
-time to equilibrium is measured on the simulated profiles of each start.
+start (straight chains) remains the fallback, and plain MD decides the profile
+error: time to equilibrium is measured on the simulated profiles of each start.
 
 Time to equilibrium: the first block of `block` steps after which the
 block-averaged brush and solvent profiles (n_bins bins over [0, Lz], mirrored

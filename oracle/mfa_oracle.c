@@ -1,7 +1,7 @@
 /*
  * mfa_oracle.c -- CPU FP64 restatement of the mfa_brush per-step MD path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the thing measured or
+ * TEST INFRASTRUCTURE ONLY (the test reference, never the thing measured or
  * shipped).  Loaded by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg; the HIP product never calls into it.
  *

@@ -1,7 +1,7 @@
 /*
  * mfa_oracle.h -- CPU FP64 restatement of the mfa_brush per-step MD path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is the test reference for the HIP path:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product (mfa_brush_amd/, include/) never links or calls it.
  *

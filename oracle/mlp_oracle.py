@@ -1,5 +1,5 @@
 """TEST INFRASTRUCTURE ONLY -- FP64 restatement of the profile surrogate
-(include/mfa_mlp.h), the checker for the BF16 MFMA kernels of
+(include/mfa_mlp.h), the test reference for the BF16 MFMA kernels of
 mfa_brush_amd/csrc/mfa_mlp.hip.  The surrogate has no counterpart in the
 reference, so this is pinned by the mathematics instead: the analytic
 gradient against central finite differences and the SiLU derivative against
