@@ -159,6 +159,47 @@ def test_full_size_force_error(oracle_mod, workload):
             assert err_max < 1e-5 and err_rms < 2e-6 and err_e < 1e-5
 
 
+def test_batched_replicas_force_error(oracle_mod):
+    """Thermostat off, the first 16 replicas of the 256-replica batch (2k-40k
+    beads; grafting density, chain length and eps_bs drawn per replica) in
+    one batched context: every replica's forces on both paths against the
+    oracle's.  The mixed path's error is stated as the largest over the
+    replicas of the max and RMS of |F - F_oracle| (relative to the
+    replica's largest and RMS oracle force) and of the energy's relative
+    error."""
+    from mfa_brush_amd.params import Flags
+    rng = np.random.default_rng(7)
+    ps, states = [], []
+    for i in range(16):
+        p, t, r0, v = configs.c4_replica(i)
+        p = p.replace(skin=0.4, flags=Flags(**{**p.flags.__dict__, "thermostat": False}))
+        r0 = r0 + rng.normal(0, 0.03, r0.shape)
+        r0[:, 0] %= p.boundary[0]
+        r0[:, 1] %= p.boundary[1]
+        ps.append(p)
+        states.append((t, r0, v))
+    ref = [oracle_mod.Oracle(p, omp=True).fluid_fluid(r0, v, t, noise=0) for p, (t, r0, v) in zip(ps, states)]
+    for precision in ("fp64", "mixed"):
+        worst = np.zeros(3)
+        with Simulation(ps, precision=precision) as s:
+            for i, (t, r0, v) in enumerate(states):
+                s.set_state(r0, v, t, isys=i)
+            s.fluid_fluid()
+            for i, (fo, eo) in enumerate(ref):
+                f = s.get_state(i)["force"]
+                e = s.energies(i)["v_fluid_fluid"]
+                no = np.linalg.norm(fo, axis=1)
+                d = np.linalg.norm(f - fo, axis=1)
+                worst = np.maximum(worst, [d.max() / no.max(), np.sqrt((d ** 2).mean() / (no ** 2).mean()),
+                                           abs(e - eo) / abs(eo)])
+        print("c4 (16 replicas, %d beads) %s: worst force err max %.2e rms %.2e, energy rel err %.2e"
+              % (sum(p.n_part for p in ps), precision, *worst))
+        if precision == "fp64":
+            assert worst[0] < 1e-11 and worst[2] < 1e-11
+        else:
+            assert worst[0] < 1e-5 and worst[1] < 2e-6 and worst[2] < 1e-5
+
+
 def test_pair_timing_same_bits():
     """Pair-kernel timing (events around each pair launch, eager launches)
     times every pair launch and leaves the trajectory's bits unchanged."""
